@@ -1,0 +1,176 @@
+/*
+ * photons_p2p.h -- native C-ABI of the MI355X (gfx950) short-range P2P
+ * gravity library libphotons_p2p.so.
+ *
+ * The library evaluates the PhotoNs PM-split short-range pair force
+ *
+ *     a_i += m * sum_j dx_ij * g(u) / r_eff^3,   u = r_eff / (2 rs),
+ *     g(u) = erfc(u) + (2/sqrt(pi)) u exp(-u^2),  zero for u > ucut,
+ *
+ * over leaf-pair interaction lists (target leaf, source leaf), the work that
+ * the reference performs in P2P_kernel / P2P_core
+ * (src/gpu_kernel.impl.h:50-157, :364-437) behind task_compute_p2p_gpu
+ * (src/gpu_entry.cu:32-209).  The reference-compatible entry points
+ * (task_compute_p2p_gpu and friends) are declared in p2p_hip.h and live in
+ * the drop-in library libp2p_hip.so, which is a thin layer over this API.
+ *
+ * Conventions: every function returns a pp2p_status (0 = success) unless
+ * stated otherwise; pointers are host pointers unless the name ends in
+ * _dev; leaf ids are 0-based; a context is used by one host thread at a
+ * time.  All device work of a context runs on one HIP stream, in call order.
+ */
+#ifndef PHOTONS_P2P_H
+#define PHOTONS_P2P_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    PP2P_OK = 0,
+    PP2P_ERR_ARG = 1,       /* invalid argument (size, range, null pointer) */
+    PP2P_ERR_DEVICE = 2,    /* HIP runtime error */
+    PP2P_ERR_ALLOC = 3,     /* device or host allocation failed */
+    PP2P_ERR_STATE = 4,     /* call out of protocol order */
+    PP2P_ERR_NOGPU = 5      /* no HIP device visible */
+} pp2p_status;
+
+/* softening: how r_eff is formed from r */
+enum {
+    PP2P_SOFT_REF = 0,      /* r_eff = max(r, eps)       src/gpu_kernel.impl.h:66-81   */
+    PP2P_SOFT_PLUMMER = 1   /* r_eff = sqrt(r^2 + eps^2) src/gpu_kernel.impl.h:274-277 */
+};
+
+/* law: how g(u) is evaluated on the device */
+enum {
+    PP2P_LAW_TABLE = 0,     /* F(s) = g/r^3 as a cubic on 16 or 32 segments per binary
+                               octave of s = r^2/rc^2, LDS table, no rsqrt (default) */
+    PP2P_LAW_POLY = 1       /* rsqrt, exp2 and a degree-12 polynomial, no table      */
+};
+
+/* frame: how positions reach the FP32 pair arithmetic */
+enum {
+    PP2P_FRAME_LEAF = 0,    /* leaf-relative FP32, FP64 per-leaf-pair offsets with
+                               optional minimum-image wrap, FP64 fold per source
+                               leaf (default) */
+    PP2P_FRAME_ABS32 = 1    /* FP32 absolute coordinates and a single FP32
+                               accumulator per particle, as the reference kernel
+                               (src/gpu_entry.cu:90-96, gpu_kernel.impl.h:405-435);
+                               for precision comparisons only */
+};
+
+/* execution form of pp2p_compute */
+enum {
+    PP2P_EXEC_QUEUE = 0,    /* one persistent launch over the cost-sorted queue */
+    PP2P_EXEC_GRID = 1,     /* one launch, one wave per work item, list order  */
+    PP2P_EXEC_PER_TARGET = 2/* one launch per target leaf (the plain form)     */
+};
+
+typedef struct {
+    int    softening;   /* PP2P_SOFT_* */
+    int    law;         /* PP2P_LAW_* */
+    int    frame;       /* PP2P_FRAME_* */
+    int    reserved;
+    double eps;         /* softening length (SoftenScale) */
+    double rs;          /* split radius (splitRadius) */
+    double ucut;        /* cutoff on u; 2.8125 in the reference (gpu_kernel.impl.h:86) */
+    double mass;        /* particle mass (MASSPART) */
+    double box;         /* periodic box length for the minimum-image offsets; 0 = open */
+} pp2p_params;
+
+/* kernel tuning: work-item shape; filled by the autotuner or by hand */
+typedef struct {
+    int waves_per_cu;     /* persistent waves per CU (0 = occupancy maximum) */
+    int split_pairs;      /* queue items of about this many pair-interactions
+                             (0 = automatic: ~8 items per persistent wave) */
+    int segs_per_octave;  /* table law: 16 or 32 segments per octave of s (0 = 32) */
+    int no_cull;          /* 1 disables the per-item j-culling (for measurement) */
+    int reserved[4];
+} pp2p_tuning;
+
+typedef struct pp2p_ctx pp2p_ctx;
+
+/* version string of the library */
+const char *pp2p_version(void);
+/* number of visible HIP devices (0 when none); never fails */
+int pp2p_device_count(void);
+/* human-readable text for the last error raised on any context of this thread */
+const char *pp2p_last_error(void);
+
+pp2p_ctx *pp2p_create(int device, int *status);
+void pp2p_destroy(pp2p_ctx *ctx);
+
+int pp2p_set_params(pp2p_ctx *ctx, const pp2p_params *params);
+int pp2p_set_tuning(pp2p_ctx *ctx, const pp2p_tuning *tuning);
+int pp2p_get_tuning(pp2p_ctx *ctx, pp2p_tuning *tuning);
+
+/*
+ * Particles and leaves.  Leaf l owns particles [leaf_ip[l], leaf_ip[l] +
+ * leaf_np[l]) (any size; ranges of different leaves need not be disjoint).  Positions are FP64 absolute
+ * coordinates, npart x 3 (AoS).  Uploads, builds the per-leaf FP64
+ * reference points and the leaf-relative FP32 positions on the device and
+ * zeroes the acceleration accumulator.
+ */
+int pp2p_set_particles_f64(pp2p_ctx *ctx, long long npart, const double *pos,
+                           int nleaf, const int *leaf_ip, const int *leaf_np);
+/*
+ * Same for integer grid positions as the reference stores them (Body.posi
+ * under INTXYZ, inc/typedef.h:29-46): three int32 at byte offset 0 of each
+ * record of `stride` bytes; position = int2pos * posi (src/gpu_entry.cu:94-96).
+ */
+int pp2p_set_particles_i32(pp2p_ctx *ctx, long long npart, const void *posi,
+                           size_t stride, double int2pos,
+                           int nleaf, const int *leaf_ip, const int *leaf_np);
+/* Per-particle levels (uint8) and the level that receives forces
+ * (ACT_OPT semantics, src/gpu_kernel.impl.h:392-396); act == NULL clears. */
+int pp2p_set_active(pp2p_ctx *ctx, const unsigned char *act, int active_level);
+
+/*
+ * Interaction list: ntask (target, source) leaf pairs.  pp2p_set_tasks
+ * replaces the list of the context; pp2p_add_tasks appends (flushes of a
+ * streamed list, LEN_TASK in src/fmm.c:500-515).  The list is not modified.
+ */
+int pp2p_set_tasks(pp2p_ctx *ctx, long long ntask, const int *task_t, const int *task_s);
+int pp2p_add_tasks(pp2p_ctx *ctx, long long ntask, const int *task_t, const int *task_s);
+
+/* Builds the target CSR and the cost-sorted work queue of the current list
+ * on the device.  pp2p_compute calls it when the list changed. */
+int pp2p_prepare(pp2p_ctx *ctx);
+
+/* Enqueues one evaluation of the current list; accelerations accumulate
+ * (+=) into the context's accumulator.  Asynchronous. */
+int pp2p_compute(pp2p_ctx *ctx, int exec);
+
+/* Zeroes the accumulator (asynchronous). */
+int pp2p_zero_acc(pp2p_ctx *ctx);
+
+/* Captures nsteps x (zero + compute(exec)) into a HIP graph and replays it
+ * nreplay times; the accumulator holds the last step's result. */
+int pp2p_run_graph(pp2p_ctx *ctx, int exec, int nsteps, int nreplay);
+
+/* Waits for the context's stream. */
+int pp2p_sync(pp2p_ctx *ctx);
+
+/* Accelerations (synchronous copy-out).  f64: npart x 3 AoS; f32: three
+ * planes of npart (x | y | z) as dev_part_acc in the reference. */
+int pp2p_get_acc_f64(pp2p_ctx *ctx, double *acc);
+int pp2p_get_acc_f32_soa(pp2p_ctx *ctx, float *acc);
+
+/* Statistics of the prepared list: number of target rows, work items,
+ * sum over tasks of n_t * n_s (src/gpu_entry.cu:50-61; active particles
+ * only when an active mask is set). */
+int pp2p_list_stats(pp2p_ctx *ctx, long long *ntask, long long *nrows,
+                    long long *nitems, long long *npairs);
+
+/* Device time in milliseconds of the force kernel launches of the last
+ * pp2p_compute (HIP events on the context stream), and of the last
+ * pp2p_prepare. */
+double pp2p_last_kernel_ms(pp2p_ctx *ctx);
+double pp2p_last_prepare_ms(pp2p_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,42 @@
+// Host entry points of the build kernels (internal to the library).
+#pragma once
+#include "p2p_device.h"
+
+namespace pp2p {
+
+struct RowArgs {
+    const int *row_tgt;   // target leaf of each CSR row
+    const int *row_off;   // nrows + 1 offsets into src
+    const int *src;       // source leaves
+    const int2 *rng;      // leaf (ip, np)
+    const int *act_cnt;   // per leaf active count (null: all particles)
+    const int *act_off;   // per leaf offset into ilist
+    int nrows;
+};
+
+hipError_t launch_leaf_frame(const double *x3, const int2 *rng, int nleaf, double inv_rc, bool frame_abs,
+                             double4 *ref, float4 *pos, hipStream_t st);
+hipError_t launch_i32_to_f64(const int *posi, long long n3, double int2pos, bool round_f32, double *x3,
+                             hipStream_t st);
+hipError_t launch_round_f32(double *x3, long long n3, hipStream_t st);
+hipError_t launch_fill_f64(double *p, long long n, double v, hipStream_t st);
+hipError_t launch_fill_i32(int *p, long long n, int v, hipStream_t st);
+
+// The functions below follow the rocPRIM convention: temp == nullptr only
+// queries the temporary storage size into *temp_bytes.
+hipError_t active_lists(const unsigned char *act, int level, const int2 *rng, int nleaf,
+                        int *cnt, int *off, int *ilist, void *temp, size_t *temp_bytes, hipStream_t st);
+hipError_t csr_sort(const int *key_in, const int *val_in, int *key_out, int *val_out, int n, int key_bits,
+                    void *temp, size_t *temp_bytes, hipStream_t st);
+hipError_t csr_rows(const int *key_sorted, int n, int *flag, int *rowidx, int *row_tgt, int *row_off,
+                    void *temp, size_t *temp_bytes, hipStream_t st);
+hipError_t rows_cost(const RowArgs &ra, unsigned long long *cost, unsigned long long *total,
+                     void *temp, size_t *temp_bytes, hipStream_t st);
+hipError_t items_count(const RowArgs &ra, long long split, int *count, int *off,
+                       void *temp, size_t *temp_bytes, hipStream_t st);
+hipError_t items_emit(const RowArgs &ra, long long split, const int *off, Item *items, unsigned *cost,
+                      hipStream_t st);
+hipError_t items_sort(const unsigned *cost, unsigned *cost_sorted, int *idx, int *idx_sorted, const Item *items,
+                      Item *items_sorted, int n, void *temp, size_t *temp_bytes, hipStream_t st);
+
+}  // namespace pp2p

@@ -1,0 +1,413 @@
+// Device construction of everything the force kernel reads besides the
+// particles: leaf frames, the target CSR of the task list and the work items.
+//
+// Target CSR (what the reference builds with thrust in src/gpu_entry.cu:
+// 132-138): a stable radix sort of (task_t, task_s) by target, then the
+// unique targets (rows) and their offsets.  Work items: each row is cut into
+// i-chunks of <= 64 particles and, for the queue form, into source ranges of
+// about `split` pair-interactions; queue items are sorted by cost, largest
+// first (longest-processing-time order for the persistent kernel).
+#include <cstring>
+
+#include <rocprim/rocprim.hpp>
+
+#include "p2p_build.h"
+
+namespace pp2p {
+
+// ---------------------------------------------------------------- frames
+
+// One wave per leaf: FP64 bounding-box centre as the leaf frame origin,
+// positions relative to it, everything scaled by 1/rc.  frame_abs: origin 0
+// and FP32 absolute coordinates (reference-style).
+__global__ __launch_bounds__(256) void k_leaf_frame(const double *__restrict__ x3, const int2 *__restrict__ rng,
+                                                    int nleaf, double inv_rc, int frame_abs,
+                                                    double4 *__restrict__ ref, float4 *__restrict__ pos)
+{
+    const int leaf = blockIdx.x * 4 + threadIdx.x / 64;
+    const int lane = threadIdx.x % 64;
+    if (leaf >= nleaf)
+        return;
+    const int2 r = rng[leaf];
+    double cx = 0.0, cy = 0.0, cz = 0.0;
+    if (!frame_abs && r.y > 0) {
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        for (int k = lane; k < r.y; k += 64) {
+            const double *p = x3 + 3 * ((long long)r.x + k);
+            for (int d = 0; d < 3; ++d) {
+                lo[d] = fmin(lo[d], p[d]);
+                hi[d] = fmax(hi[d], p[d]);
+            }
+        }
+        for (int o = 32; o > 0; o >>= 1)
+            for (int d = 0; d < 3; ++d) {
+                lo[d] = fmin(lo[d], __shfl_xor(lo[d], o, 64));
+                hi[d] = fmax(hi[d], __shfl_xor(hi[d], o, 64));
+            }
+        cx = 0.5 * (lo[0] + hi[0]);
+        cy = 0.5 * (lo[1] + hi[1]);
+        cz = 0.5 * (lo[2] + hi[2]);
+    }
+    if (lane == 0)
+        ref[leaf] = make_double4(cx * inv_rc, cy * inv_rc, cz * inv_rc, 0.0);
+    for (int k = lane; k < r.y; k += 64) {
+        const long long i = (long long)r.x + k;
+        const double *p = x3 + 3 * i;
+        pos[i] = make_float4((float)((p[0] - cx) * inv_rc), (float)((p[1] - cy) * inv_rc),
+                             (float)((p[2] - cz) * inv_rc), 0.f);
+    }
+}
+
+__global__ void k_i32_to_f64(const int *__restrict__ posi, long long n3, double int2pos, double *__restrict__ x3)
+{
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n3)
+        x3[k] = int2pos * (double)posi[k];
+}
+
+// abs frame in the reference's quantisation: float(INT2POS * posi)
+// (src/gpu_entry.cu:94-96), then scaled
+__global__ void k_f64_round_f32(double *__restrict__ x3, long long n3)
+{
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n3)
+        x3[k] = (double)(float)x3[k];
+}
+
+// ---------------------------------------------------------------- fills
+
+__global__ void k_fill_f64(double *p, long long n, double v)
+{
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long long)gridDim.x * blockDim.x)
+        p[k] = v;
+}
+
+__global__ void k_fill_i32(int *p, long long n, int v)
+{
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long long)gridDim.x * blockDim.x)
+        p[k] = v;
+}
+
+static int fill_grid(long long n)
+{
+    long long g = (n + 255) / 256;
+    return (int)(g < 4096 ? (g > 0 ? g : 1) : 4096);
+}
+
+// ---------------------------------------------------------------- CSR
+
+__global__ void k_row_flags(const int *__restrict__ key, int n, int *__restrict__ flag)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n)
+        flag[k] = (k == 0 || key[k] != key[k - 1]) ? 1 : 0;
+}
+
+// rowidx: inclusive scan of the flags
+__global__ void k_row_emit(const int *__restrict__ key, const int *__restrict__ flag, const int *__restrict__ rowidx,
+                           int n, int *__restrict__ row_tgt, int *__restrict__ row_off)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n)
+        return;
+    if (flag[k]) {
+        row_tgt[rowidx[k] - 1] = key[k];
+        row_off[rowidx[k] - 1] = k;
+    }
+    if (k == n - 1)
+        row_off[rowidx[k]] = n;
+}
+
+// ---------------------------------------------------------------- active lists
+
+// One wave per leaf: number of particles with act == level.
+__global__ __launch_bounds__(256) void k_active_count(const unsigned char *__restrict__ act, int level,
+                                                      const int2 *__restrict__ rng, int nleaf, int *__restrict__ cnt)
+{
+    const int leaf = blockIdx.x * 4 + threadIdx.x / 64;
+    const int lane = threadIdx.x % 64;
+    if (leaf >= nleaf)
+        return;
+    const int2 r = rng[leaf];
+    int c = 0;
+    for (int k = lane; k < r.y; k += 64)
+        c += act[(long long)r.x + k] == (unsigned char)level;
+    for (int o = 32; o > 0; o >>= 1)
+        c += __shfl_xor(c, o, 64);
+    if (lane == 0)
+        cnt[leaf] = c;
+}
+
+// off: exclusive scan of the counts; ilist gets the active indices in order.
+__global__ __launch_bounds__(256) void k_active_emit(const unsigned char *__restrict__ act, int level,
+                                                     const int2 *__restrict__ rng, int nleaf,
+                                                     const int *__restrict__ off, int *__restrict__ ilist)
+{
+    const int leaf = blockIdx.x * 4 + threadIdx.x / 64;
+    const int lane = threadIdx.x % 64;
+    if (leaf >= nleaf)
+        return;
+    const int2 r = rng[leaf];
+    int base = off[leaf];
+    const unsigned long long below = (1ull << lane) - 1ull;
+    for (int k0 = 0; k0 < r.y; k0 += 64) {
+        const bool a = k0 + lane < r.y && act[(long long)r.x + k0 + lane] == (unsigned char)level;
+        const unsigned long long m = __ballot(a);
+        if (a)
+            ilist[base + __popcll(m & below)] = r.x + k0 + lane;
+        base += __popcll(m);
+    }
+}
+
+// ---------------------------------------------------------------- items
+
+struct RowView {
+    const int *row_tgt, *row_off, *src;
+    const int2 *rng;
+    const int *act_cnt, *act_off;   // null: all particles of the target
+};
+
+__device__ __forceinline__ void row_i(const RowView &v, int tgt, int &ibeg, int &ni)
+{
+    if (v.act_cnt) {
+        ibeg = v.act_off[tgt];
+        ni = v.act_cnt[tgt];
+    } else {
+        const int2 r = v.rng[tgt];
+        ibeg = r.x;
+        ni = r.y;
+    }
+}
+
+// pairs of every row (sum over its tasks of n_i * n_s)
+__global__ void k_row_cost(RowView v, int nrows, unsigned long long *__restrict__ cost)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrows)
+        return;
+    int ibeg, ni;
+    row_i(v, v.row_tgt[r], ibeg, ni);
+    unsigned long long ns = 0;
+    for (int k = v.row_off[r]; k < v.row_off[r + 1]; ++k)
+        ns += (unsigned)v.rng[v.src[k]].y;
+    cost[r] = ns * (unsigned long long)ni;
+}
+
+// Source ranges of a row: greedy cut when the running pairs of one i-chunk
+// (min(ni, 64) i-particles) would exceed `split`; split <= 0: one range.
+template <bool EMIT>
+__device__ int row_ranges(const RowView &v, int r, long long split, int tgt, int ibeg, int ni,
+                          Item *items, unsigned *cost, int out0)
+{
+    const int ichunks = (ni + kItemMaxI - 1) / kItemMaxI;
+    if (ni == 0)
+        return 0;
+    const long long wi = ni < kItemMaxI ? ni : kItemMaxI;
+    const int k0 = v.row_off[r], k1 = v.row_off[r + 1];
+    int nr = 0, out = out0;
+    int beg = k0;
+    long long acc = 0;
+    for (int k = k0; k < k1; ++k) {
+        const long long c = wi * (long long)v.rng[v.src[k]].y;
+        if (split > 0 && k > beg && acc + c > split) {
+            if (EMIT)
+                for (int q = 0; q < ichunks; ++q) {
+                    const int nq = min(kItemMaxI, ni - q * kItemMaxI);
+                    items[out] = Item{tgt, ibeg + q * kItemMaxI, nq, r, beg, k, 0, 0};
+                    if (cost)
+                        cost[out] = (unsigned)min(acc / wi * nq, 0xffffffffll);
+                    ++out;
+                }
+            ++nr;
+            beg = k;
+            acc = 0;
+        }
+        acc += c;
+    }
+    if (EMIT)
+        for (int q = 0; q < ichunks; ++q) {
+            const int nq = min(kItemMaxI, ni - q * kItemMaxI);
+            items[out] = Item{tgt, ibeg + q * kItemMaxI, nq, r, beg, k1, 0, 0};
+            if (cost)
+                cost[out] = (unsigned)min(acc / wi * nq, 0xffffffffll);
+            ++out;
+        }
+    ++nr;
+    return nr * ichunks;
+}
+
+__global__ void k_items_count(RowView v, int nrows, long long split, int *__restrict__ count)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrows)
+        return;
+    int ibeg, ni;
+    const int tgt = v.row_tgt[r];
+    row_i(v, tgt, ibeg, ni);
+    count[r] = row_ranges<false>(v, r, split, tgt, ibeg, ni, nullptr, nullptr, 0);
+}
+
+// off: exclusive scan of the counts (nrows + 1 entries)
+__global__ void k_items_emit(RowView v, int nrows, long long split, const int *__restrict__ off,
+                             Item *__restrict__ items, unsigned *__restrict__ cost)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrows)
+        return;
+    int ibeg, ni;
+    const int tgt = v.row_tgt[r];
+    row_i(v, tgt, ibeg, ni);
+    row_ranges<true>(v, r, split, tgt, ibeg, ni, items, cost, off[r]);
+}
+
+__global__ void k_gather_items(const Item *__restrict__ in, const int *__restrict__ order, int n, Item *__restrict__ out)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n)
+        out[k] = in[order[k]];
+}
+
+__global__ void k_iota(int *p, int n)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n)
+        p[k] = k;
+}
+
+// ---------------------------------------------------------------- host side
+
+static int grid_of(long long n, int b = 256) { return (int)((n + b - 1) / b); }
+
+hipError_t launch_leaf_frame(const double *x3, const int2 *rng, int nleaf, double inv_rc, bool frame_abs,
+                             double4 *ref, float4 *pos, hipStream_t st)
+{
+    if (nleaf <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_leaf_frame, dim3(grid_of(nleaf, 4)), dim3(256), 0, st, x3, rng, nleaf, inv_rc,
+                       frame_abs ? 1 : 0, ref, pos);
+    return hipGetLastError();
+}
+
+hipError_t launch_i32_to_f64(const int *posi, long long n3, double int2pos, bool round_f32, double *x3, hipStream_t st)
+{
+    if (n3 <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_i32_to_f64, dim3(grid_of(n3)), dim3(256), 0, st, posi, n3, int2pos, x3);
+    if (round_f32)
+        hipLaunchKernelGGL(k_f64_round_f32, dim3(grid_of(n3)), dim3(256), 0, st, x3, n3);
+    return hipGetLastError();
+}
+
+hipError_t launch_round_f32(double *x3, long long n3, hipStream_t st)
+{
+    if (n3 <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_f64_round_f32, dim3(grid_of(n3)), dim3(256), 0, st, x3, n3);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_f64(double *p, long long n, double v, hipStream_t st)
+{
+    if (n <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_fill_f64, dim3(fill_grid(n)), dim3(256), 0, st, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_i32(int *p, long long n, int v, hipStream_t st)
+{
+    if (n <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_fill_i32, dim3(fill_grid(n)), dim3(256), 0, st, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t active_lists(const unsigned char *act, int level, const int2 *rng, int nleaf,
+                        int *cnt, int *off, int *ilist, void *temp, size_t *temp_bytes, hipStream_t st)
+{
+    if (temp == nullptr) {
+        return rocprim::exclusive_scan(nullptr, *temp_bytes, cnt, off, 0, (size_t)nleaf + 1,
+                                       rocprim::plus<int>(), st);
+    }
+    hipLaunchKernelGGL(k_active_count, dim3(grid_of(nleaf, 4)), dim3(256), 0, st, act, level, rng, nleaf, cnt);
+    hipError_t e = rocprim::exclusive_scan(temp, *temp_bytes, cnt, off, 0, (size_t)nleaf + 1,
+                                           rocprim::plus<int>(), st);
+    if (e != hipSuccess)
+        return e;
+    hipLaunchKernelGGL(k_active_emit, dim3(grid_of(nleaf, 4)), dim3(256), 0, st, act, level, rng, nleaf, off, ilist);
+    return hipGetLastError();
+}
+
+hipError_t csr_sort(const int *key_in, const int *val_in, int *key_out, int *val_out, int n, int key_bits,
+                    void *temp, size_t *temp_bytes, hipStream_t st)
+{
+    return rocprim::radix_sort_pairs(temp, *temp_bytes, key_in, key_out, val_in, val_out, (size_t)n,
+                                     0, key_bits, st);
+}
+
+hipError_t csr_rows(const int *key_sorted, int n, int *flag, int *rowidx, int *row_tgt, int *row_off,
+                    void *temp, size_t *temp_bytes, hipStream_t st)
+{
+    if (temp == nullptr)
+        return rocprim::inclusive_scan(nullptr, *temp_bytes, flag, rowidx, (size_t)n, rocprim::plus<int>(), st);
+    hipLaunchKernelGGL(k_row_flags, dim3(grid_of(n)), dim3(256), 0, st, key_sorted, n, flag);
+    hipError_t e = rocprim::inclusive_scan(temp, *temp_bytes, flag, rowidx, (size_t)n, rocprim::plus<int>(), st);
+    if (e != hipSuccess)
+        return e;
+    hipLaunchKernelGGL(k_row_emit, dim3(grid_of(n)), dim3(256), 0, st, key_sorted, flag, rowidx, n, row_tgt, row_off);
+    return hipGetLastError();
+}
+
+hipError_t rows_cost(const RowArgs &ra, unsigned long long *cost, unsigned long long *total,
+                     void *temp, size_t *temp_bytes, hipStream_t st)
+{
+    if (temp == nullptr)
+        return rocprim::reduce(nullptr, *temp_bytes, cost, total, 0ull, (size_t)ra.nrows,
+                               rocprim::plus<unsigned long long>(), st);
+    RowView v{ra.row_tgt, ra.row_off, ra.src, ra.rng, ra.act_cnt, ra.act_off};
+    hipLaunchKernelGGL(k_row_cost, dim3(grid_of(ra.nrows)), dim3(256), 0, st, v, ra.nrows, cost);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return e;
+    return rocprim::reduce(temp, *temp_bytes, cost, total, 0ull, (size_t)ra.nrows,
+                           rocprim::plus<unsigned long long>(), st);
+}
+
+hipError_t items_count(const RowArgs &ra, long long split, int *count, int *off,
+                       void *temp, size_t *temp_bytes, hipStream_t st)
+{
+    if (temp == nullptr)
+        return rocprim::exclusive_scan(nullptr, *temp_bytes, count, off, 0, (size_t)ra.nrows + 1,
+                                       rocprim::plus<int>(), st);
+    RowView v{ra.row_tgt, ra.row_off, ra.src, ra.rng, ra.act_cnt, ra.act_off};
+    hipLaunchKernelGGL(k_items_count, dim3(grid_of(ra.nrows)), dim3(256), 0, st, v, ra.nrows, split, count);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return e;
+    return rocprim::exclusive_scan(temp, *temp_bytes, count, off, 0, (size_t)ra.nrows + 1,
+                                   rocprim::plus<int>(), st);
+}
+
+hipError_t items_emit(const RowArgs &ra, long long split, const int *off, Item *items, unsigned *cost, hipStream_t st)
+{
+    RowView v{ra.row_tgt, ra.row_off, ra.src, ra.rng, ra.act_cnt, ra.act_off};
+    hipLaunchKernelGGL(k_items_emit, dim3(grid_of(ra.nrows)), dim3(256), 0, st, v, ra.nrows, split, off, items, cost);
+    return hipGetLastError();
+}
+
+hipError_t items_sort(const unsigned *cost, unsigned *cost_sorted, int *idx, int *idx_sorted, const Item *items,
+                      Item *items_sorted, int n, void *temp, size_t *temp_bytes, hipStream_t st)
+{
+    if (temp == nullptr)
+        return rocprim::radix_sort_pairs_desc(nullptr, *temp_bytes, cost, cost_sorted, idx, idx_sorted,
+                                              (size_t)n, 0, 32, st);
+    hipLaunchKernelGGL(k_iota, dim3(grid_of(n)), dim3(256), 0, st, idx, n);
+    hipError_t e = rocprim::radix_sort_pairs_desc(temp, *temp_bytes, cost, cost_sorted, idx, idx_sorted,
+                                                  (size_t)n, 0, 32, st);
+    if (e != hipSuccess)
+        return e;
+    hipLaunchKernelGGL(k_gather_items, dim3(grid_of(n)), dim3(256), 0, st, items, idx_sorted, n, items_sorted);
+    return hipGetLastError();
+}
+
+}  // namespace pp2p

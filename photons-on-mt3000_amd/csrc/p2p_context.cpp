@@ -1,0 +1,825 @@
+// Host context of libphotons_p2p.so: device buffers, uploads, the prepare
+// step (target CSR + work items) and the launches.  One HIP stream per
+// context; every call returns a pp2p_status and records a message on error.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/photons_p2p.h"
+#include "forcelaw.h"
+#include "p2p_build.h"
+#include "p2p_launch.h"
+
+using namespace pp2p;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                       \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return fail(PP2P_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// grow-only device buffer
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t cap = 0;
+    hipError_t reserve(size_t n)
+    {
+        if (n <= cap)
+            return hipSuccess;
+        size_t want = std::max(n, cap + cap / 4);
+        if (p)
+            (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        hipError_t e = hipMalloc((void **)&p, sizeof(T) * std::max<size_t>(want, 1));
+        if (e == hipSuccess)
+            cap = want;
+        return e;
+    }
+    void release()
+    {
+        if (p)
+            (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+}  // namespace
+
+struct pp2p_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_k0 = nullptr, ev_k1 = nullptr, ev_p0 = nullptr, ev_p1 = nullptr;
+    int num_cu = 0;
+
+    pp2p_params params{};
+    bool have_params = false;
+    pp2p_tuning tuning{};
+    int segs = 32;
+    LawSetup law;
+    DevBuf<float4> table;
+
+    // particles
+    long long npart = 0;
+    int nleaf = 0;
+    bool have_particles = false;
+    DevBuf<float4> pos;
+    DevBuf<double4> ref;
+    DevBuf<int2> rng;
+    DevBuf<double> acc;
+    DevBuf<double> x3;          // FP64 staging of positions
+    DevBuf<int> posi;           // int32 staging
+    std::vector<int2> h_rng;
+
+    // active subset
+    bool has_act = false;
+    int act_level = 0;
+    DevBuf<unsigned char> act;
+    DevBuf<int> act_cnt, act_off, ilist;
+
+    // task list (device copy, appended by pp2p_add_tasks)
+    long long ntask = 0;
+    DevBuf<int> task_t, task_s;
+    long long pairs_estimate = 0;   // sum n_t n_s of the list, host side
+
+    // CSR
+    DevBuf<int> key_sorted, src, flag, rowidx, row_tgt, row_off;
+    int nrows = 0;
+    bool csr_ready = false;
+
+    // items
+    DevBuf<unsigned long long> row_cost, total;
+    DevBuf<int> item_count, item_off;
+    DevBuf<Item> items_plain, items_q_raw, items_q;
+    DevBuf<unsigned> cost, cost_sorted;
+    DevBuf<int> idx, idx_sorted;
+    int nitems_plain = 0, nitems_q = 0;
+    bool plain_ready = false, queue_ready = false;
+    std::vector<int> h_item_off_plain;
+    unsigned long long list_pairs = 0;
+
+    DevBuf<unsigned> counter;
+    DevBuf<unsigned char> temp;
+
+    float last_kernel_ms = 0.f, last_prepare_ms = 0.f;
+    bool kernel_timed = false, prepare_timed = false;
+};
+
+static int check_ctx(pp2p_ctx *c)
+{
+    if (!c)
+        return fail(PP2P_ERR_ARG, "null context");
+    hipError_t e = hipSetDevice(c->device);
+    if (e != hipSuccess)
+        return fail(PP2P_ERR_DEVICE, std::string("hipSetDevice: ") + hipGetErrorString(e));
+    return PP2P_OK;
+}
+
+static int ensure_temp(pp2p_ctx *c, size_t bytes)
+{
+    HIPCHK(c->temp.reserve(std::max<size_t>(bytes, 256)));
+    return PP2P_OK;
+}
+
+extern "C" {
+
+const char *pp2p_version(void) { return "photons_p2p 0.1 (gfx950)"; }
+
+int pp2p_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess)
+        return 0;
+    return n;
+}
+
+const char *pp2p_last_error(void) { return g_last_error.c_str(); }
+
+pp2p_ctx *pp2p_create(int device, int *status)
+{
+    int st = PP2P_OK;
+    pp2p_ctx *c = nullptr;
+    int n = pp2p_device_count();
+    if (n <= 0) {
+        st = fail(PP2P_ERR_NOGPU, "no HIP device visible");
+    } else if (device < 0 || device >= n) {
+        st = fail(PP2P_ERR_ARG, "device index out of range");
+    } else {
+        c = new pp2p_ctx();
+        c->device = device;
+        hipDeviceProp_t prop;
+        if (hipSetDevice(device) != hipSuccess || hipGetDeviceProperties(&prop, device) != hipSuccess ||
+            hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreate(&c->ev_k0) != hipSuccess || hipEventCreate(&c->ev_k1) != hipSuccess ||
+            hipEventCreate(&c->ev_p0) != hipSuccess || hipEventCreate(&c->ev_p1) != hipSuccess ||
+            c->counter.reserve(1) != hipSuccess) {
+            st = fail(PP2P_ERR_DEVICE, "context creation failed");
+            pp2p_destroy(c);
+            c = nullptr;
+        } else {
+            c->num_cu = prop.multiProcessorCount;
+        }
+    }
+    if (status)
+        *status = st;
+    return c;
+}
+
+void pp2p_destroy(pp2p_ctx *c)
+{
+    if (!c)
+        return;
+    (void)hipSetDevice(c->device);
+    if (c->stream)
+        (void)hipStreamSynchronize(c->stream);
+    c->table.release(); c->pos.release(); c->ref.release(); c->rng.release(); c->acc.release();
+    c->x3.release(); c->posi.release(); c->act.release(); c->act_cnt.release(); c->act_off.release();
+    c->ilist.release(); c->task_t.release(); c->task_s.release(); c->key_sorted.release(); c->src.release();
+    c->flag.release(); c->rowidx.release(); c->row_tgt.release(); c->row_off.release(); c->row_cost.release();
+    c->total.release(); c->item_count.release(); c->item_off.release(); c->items_plain.release();
+    c->items_q_raw.release(); c->items_q.release(); c->cost.release(); c->cost_sorted.release();
+    c->idx.release(); c->idx_sorted.release(); c->counter.release(); c->temp.release();
+    if (c->ev_k0) (void)hipEventDestroy(c->ev_k0);
+    if (c->ev_k1) (void)hipEventDestroy(c->ev_k1);
+    if (c->ev_p0) (void)hipEventDestroy(c->ev_p0);
+    if (c->ev_p1) (void)hipEventDestroy(c->ev_p1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int pp2p_set_tuning(pp2p_ctx *c, const pp2p_tuning *t)
+{
+    int st = check_ctx(c);
+    if (st)
+        return st;
+    if (!t || t->waves_per_cu < 0 || t->split_pairs < 0)
+        return fail(PP2P_ERR_ARG, "invalid tuning");
+    const int segs = t->segs_per_octave;
+    if (segs != 0 && segs != 16 && segs != 32)
+        return fail(PP2P_ERR_ARG, "tuning: segments per octave must be 0, 16 or 32");
+    c->tuning = *t;
+    const int new_segs = segs ? segs : 32;
+    if (new_segs != c->segs) {
+        c->segs = new_segs;
+        if (c->have_params) {
+            pp2p_params p = c->params;
+            c->have_params = false;
+            return pp2p_set_params(c, &p);
+        }
+    }
+    c->queue_ready = false;
+    return PP2P_OK;
+}
+
+int pp2p_get_tuning(pp2p_ctx *c, pp2p_tuning *t)
+{
+    if (!c || !t)
+        return fail(PP2P_ERR_ARG, "null argument");
+    *t = c->tuning;
+    return PP2P_OK;
+}
+
+int pp2p_set_params(pp2p_ctx *c, const pp2p_params *p)
+{
+    int st = check_ctx(c);
+    if (st)
+        return st;
+    if (!p)
+        return fail(PP2P_ERR_ARG, "null params");
+    if (p->law != PP2P_LAW_TABLE && p->law != PP2P_LAW_POLY)
+        return fail(PP2P_ERR_ARG, "unknown law");
+    if (p->frame != PP2P_FRAME_LEAF && p->frame != PP2P_FRAME_ABS32)
+        return fail(PP2P_ERR_ARG, "unknown frame");
+    if (!(p->mass > 0.0) || !(p->box >= 0.0))
+        return fail(PP2P_ERR_ARG, "mass must be > 0 and box >= 0");
+    const char *err = nullptr;
+    LawSetup ls;
+    if (!law_setup(p->softening, p->eps, p->rs, p->ucut, c->segs, &ls, &err))
+        return fail(PP2P_ERR_ARG, err);
+    if (p->box > 0.0 && p->box < 2.0 * ls.rc)
+        return fail(PP2P_ERR_ARG, "periodic box shorter than twice the cutoff radius");
+    // frame change invalidates the converted positions
+    const bool frame_changed = c->have_params && (c->params.frame != p->frame || c->law.rc != ls.rc);
+    c->law = ls;
+    c->params = *p;
+    c->have_params = true;
+    HIPCHK(c->table.reserve(ls.table_len));
+    HIPCHK(hipMemcpyAsync(c->table.p, ls.table.data(), sizeof(float) * ls.table.size(), hipMemcpyHostToDevice,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (frame_changed)
+        c->have_particles = false;
+    return PP2P_OK;
+}
+
+static int upload_leaves(pp2p_ctx *c, long long npart, int nleaf, const int *leaf_ip, const int *leaf_np)
+{
+    if (nleaf < 0 || (nleaf > 0 && (!leaf_ip || !leaf_np)))
+        return fail(PP2P_ERR_ARG, "invalid leaf arrays");
+    c->h_rng.resize(nleaf);
+    for (int l = 0; l < nleaf; ++l) {
+        if (leaf_np[l] < 0 || leaf_ip[l] < 0 || (long long)leaf_ip[l] + leaf_np[l] > npart)
+            return fail(PP2P_ERR_ARG, "leaf " + std::to_string(l) + " outside the particle range");
+        c->h_rng[l] = make_int2(leaf_ip[l], leaf_np[l]);
+    }
+    HIPCHK(c->rng.reserve(std::max(nleaf, 1)));
+    if (nleaf)
+        HIPCHK(hipMemcpyAsync(c->rng.p, c->h_rng.data(), sizeof(int2) * nleaf, hipMemcpyHostToDevice, c->stream));
+    return PP2P_OK;
+}
+
+static int finish_particles(pp2p_ctx *c, long long npart, int nleaf)
+{
+    HIPCHK(c->pos.reserve(std::max<long long>(npart, 1)));
+    HIPCHK(c->ref.reserve(std::max(nleaf, 1)));
+    HIPCHK(c->acc.reserve(std::max<long long>(3 * npart, 1)));
+    const bool abs = c->params.frame == PP2P_FRAME_ABS32;
+    if (abs)
+        HIPCHK(launch_round_f32(c->x3.p, 3 * npart, c->stream));
+    HIPCHK(launch_leaf_frame(c->x3.p, c->rng.p, nleaf, 1.0 / c->law.rc, abs, c->ref.p, c->pos.p, c->stream));
+    HIPCHK(launch_fill_f64(c->acc.p, 3 * npart, 0.0, c->stream));
+    c->npart = npart;
+    c->nleaf = nleaf;
+    c->have_particles = true;
+    c->has_act = false;
+    c->ntask = 0;
+    c->csr_ready = c->plain_ready = c->queue_ready = false;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return PP2P_OK;
+}
+
+int pp2p_set_particles_f64(pp2p_ctx *c, long long npart, const double *pos, int nleaf, const int *leaf_ip,
+                           const int *leaf_np)
+{
+    int st = check_ctx(c);
+    if (st)
+        return st;
+    if (!c->have_params)
+        return fail(PP2P_ERR_STATE, "set_params before set_particles");
+    if (npart < 0 || npart > (1ll << 31) - 1 || (npart > 0 && !pos))
+        return fail(PP2P_ERR_ARG, "invalid particle array");
+    if ((st = upload_leaves(c, npart, nleaf, leaf_ip, leaf_np)))
+        return st;
+    HIPCHK(c->x3.reserve(std::max<long long>(3 * npart, 1)));
+    if (npart)
+        HIPCHK(hipMemcpyAsync(c->x3.p, pos, sizeof(double) * 3 * npart, hipMemcpyHostToDevice, c->stream));
+    return finish_particles(c, npart, nleaf);
+}
+
+int pp2p_set_particles_i32(pp2p_ctx *c, long long npart, const void *posi, size_t stride, double int2pos, int nleaf,
+                           const int *leaf_ip, const int *leaf_np)
+{
+    int st = check_ctx(c);
+    if (st)
+        return st;
+    if (!c->have_params)
+        return fail(PP2P_ERR_STATE, "set_params before set_particles");
+    if (npart < 0 || npart > (1ll << 31) - 1 || (npart > 0 && !posi) || stride < 12 || !(int2pos > 0.0))
+        return fail(PP2P_ERR_ARG, "invalid particle array");
+    if ((st = upload_leaves(c, npart, nleaf, leaf_ip, leaf_np)))
+        return st;
+    HIPCHK(c->posi.reserve(std::max<long long>(3 * npart, 1)));
+    HIPCHK(c->x3.reserve(std::max<long long>(3 * npart, 1)));
+    if (npart) {
+        HIPCHK(hipMemcpy2DAsync(c->posi.p, 12, posi, stride, 12, (size_t)npart, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(launch_i32_to_f64(c->posi.p, 3 * npart, int2pos, false, c->x3.p, c->stream));
+    }
+    return finish_particles(c, npart, nleaf);
+}
+
+int pp2p_set_active(pp2p_ctx *c, const unsigned char *a, int level)
+{
+    int st = check_ctx(c);
+    if (st)
+        return st;
+    if (!c->have_particles)
+        return fail(PP2P_ERR_STATE, "set_particles before set_active");
+    c->plain_ready = c->queue_ready = false;
+    if (!a) {
+        c->has_act = false;
+        return PP2P_OK;
+    }
+    if (level < 0 || level > 255)
+        return fail(PP2P_ERR_ARG, "active level outside [0, 255]");
+    HIPCHK(c->act.reserve(std::max<long long>(c->npart, 1)));
+    HIPCHK(c->act_cnt.reserve(c->nleaf + 1));
+    HIPCHK(c->act_off.reserve(c->nleaf + 1));
+    HIPCHK(c->ilist.reserve(std::max<long long>(c->npart, 1)));
+    if (c->npart)
+        HIPCHK(hipMemcpyAsync(c->act.p, a, c->npart, hipMemcpyHostToDevice, c->stream));
+    size_t tb = 0;
+    HIPCHK(active_lists(c->act.p, level, c->rng.p, c->nleaf, c->act_cnt.p, c->act_off.p, c->ilist.p, nullptr, &tb,
+                        c->stream));
+    if ((st = ensure_temp(c, tb)))
+        return st;
+    tb = c->temp.cap;
+    HIPCHK(launch_fill_i32(c->act_cnt.p + c->nleaf, 1, 0, c->stream));
+    HIPCHK(active_lists(c->act.p, level, c->rng.p, c->nleaf, c->act_cnt.p, c->act_off.p, c->ilist.p, c->temp.p, &tb,
+                        c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->has_act = true;
+    c->act_level = level;
+    return PP2P_OK;
+}
+
+static int append_tasks(pp2p_ctx *c, long long n, const int *tt, const int *ts)
+{
+    if (n < 0 || (n > 0 && (!tt || !ts)))
+        return fail(PP2P_ERR_ARG, "invalid task arrays");
+    if (c->ntask + n > (1ll << 31) - 1)
+        return fail(PP2P_ERR_ARG, "more than 2^31-1 tasks in one list");
+    for (long long k = 0; k < n; ++k)
+        if (tt[k] < 0 || tt[k] >= c->nleaf || ts[k] < 0 || ts[k] >= c->nleaf)
+            return fail(PP2P_ERR_ARG, "task " + std::to_string(k) + " names a leaf outside [0, nleaf)");
+    const size_t need = (size_t)(c->ntask + n);
+    if (need > c->task_t.cap) {
+        // keep the existing tasks across the grow
+        DevBuf<int> nt, ns;
+        HIPCHK(nt.reserve(need));
+        HIPCHK(ns.reserve(need));
+        if (c->ntask) {
+            HIPCHK(hipMemcpyAsync(nt.p, c->task_t.p, sizeof(int) * c->ntask, hipMemcpyDeviceToDevice, c->stream));
+            HIPCHK(hipMemcpyAsync(ns.p, c->task_s.p, sizeof(int) * c->ntask, hipMemcpyDeviceToDevice, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+        }
+        c->task_t.release();
+        c->task_s.release();
+        c->task_t = nt;
+        c->task_s = ns;
+    }
+    if (n) {
+        HIPCHK(hipMemcpyAsync(c->task_t.p + c->ntask, tt, sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(c->task_s.p + c->ntask, ts, sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));   // the caller may reuse its buffers on return
+    }
+    c->ntask += n;
+    c->csr_ready = c->plain_ready = c->queue_ready = false;
+    return PP2P_OK;
+}
+
+int pp2p_set_tasks(pp2p_ctx *c, long long n, const int *tt, const int *ts)
+{
+    int st = check_ctx(c);
+    if (st)
+        return st;
+    if (!c->have_particles)
+        return fail(PP2P_ERR_STATE, "set_particles before set_tasks");
+    c->ntask = 0;
+    return append_tasks(c, n, tt, ts);
+}
+
+int pp2p_add_tasks(pp2p_ctx *c, long long n, const int *tt, const int *ts)
+{
+    int st = check_ctx(c);
+    if (st)
+        return st;
+    if (!c->have_particles)
+        return fail(PP2P_ERR_STATE, "set_particles before add_tasks");
+    return append_tasks(c, n, tt, ts);
+}
+
+static int build_csr(pp2p_ctx *c)
+{
+    const int n = (int)c->ntask;
+    HIPCHK(c->key_sorted.reserve(std::max(n, 1)));
+    HIPCHK(c->src.reserve(std::max(n, 1)));
+    HIPCHK(c->flag.reserve(std::max(n, 1)));
+    HIPCHK(c->rowidx.reserve(std::max(n, 1)));
+    HIPCHK(c->row_tgt.reserve(std::max(n, 1)));
+    HIPCHK(c->row_off.reserve(n + 1));
+    c->nrows = 0;
+    if (n == 0) {
+        c->csr_ready = true;
+        return PP2P_OK;
+    }
+    int bits = 1;
+    while ((1ll << bits) < (long long)c->nleaf)
+        ++bits;
+    size_t tb1 = 0, tb2 = 0;
+    HIPCHK(csr_sort(c->task_t.p, c->task_s.p, c->key_sorted.p, c->src.p, n, bits, nullptr, &tb1, c->stream));
+    HIPCHK(csr_rows(c->key_sorted.p, n, c->flag.p, c->rowidx.p, c->row_tgt.p, c->row_off.p, nullptr, &tb2,
+                    c->stream));
+    int st = ensure_temp(c, std::max(tb1, tb2));
+    if (st)
+        return st;
+    tb1 = tb2 = c->temp.cap;
+    HIPCHK(csr_sort(c->task_t.p, c->task_s.p, c->key_sorted.p, c->src.p, n, bits, c->temp.p, &tb1, c->stream));
+    HIPCHK(csr_rows(c->key_sorted.p, n, c->flag.p, c->rowidx.p, c->row_tgt.p, c->row_off.p, c->temp.p, &tb2,
+                    c->stream));
+    int rows = 0;
+    HIPCHK(hipMemcpyAsync(&rows, c->rowidx.p + n - 1, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->nrows = rows;
+    c->csr_ready = true;
+    return PP2P_OK;
+}
+
+static RowArgs row_args(pp2p_ctx *c)
+{
+    RowArgs ra;
+    ra.row_tgt = c->row_tgt.p;
+    ra.row_off = c->row_off.p;
+    ra.src = c->src.p;
+    ra.rng = c->rng.p;
+    ra.act_cnt = c->has_act ? c->act_cnt.p : nullptr;
+    ra.act_off = c->has_act ? c->act_off.p : nullptr;
+    ra.nrows = c->nrows;
+    return ra;
+}
+
+static ForceVariant variant(pp2p_ctx *c, bool queue)
+{
+    ForceVariant v;
+    v.law = c->params.law == PP2P_LAW_POLY ? LAW_POLY : LAW_LOGTAB;
+    v.soft = c->params.softening == PP2P_SOFT_PLUMMER ? SOFT_PLUMMER : SOFT_REF;
+    v.fold64 = c->params.frame == PP2P_FRAME_LEAF;
+    v.indirect = c->has_act;
+    v.queue = queue;
+    return v;
+}
+
+// total persistent waves of the queue form
+static int queue_groups(pp2p_ctx *c)
+{
+    int per_cu = 0;
+    ForceVariant v = variant(c, true);
+    if (force_occupancy(v, c->law.table_len, &per_cu) != hipSuccess || per_cu <= 0)
+        per_cu = 2;
+    if (c->tuning.waves_per_cu > 0)
+        per_cu = std::max(1, std::min(per_cu, (c->tuning.waves_per_cu + kWavesPerGroup - 1) / kWavesPerGroup));
+    return per_cu * c->num_cu;
+}
+
+static int build_items(pp2p_ctx *c, bool queue)
+{
+    const RowArgs ra = row_args(c);
+    const int nr = c->nrows;
+    HIPCHK(c->row_cost.reserve(std::max(nr, 1)));
+    HIPCHK(c->total.reserve(1));
+    HIPCHK(c->item_count.reserve(nr + 1));
+    HIPCHK(c->item_off.reserve(nr + 1));
+    if (nr == 0) {
+        (queue ? c->nitems_q : c->nitems_plain) = 0;
+        c->list_pairs = 0;
+        if (!queue)
+            c->h_item_off_plain.assign(1, 0);
+        return PP2P_OK;
+    }
+    size_t tb1 = 0, tb2 = 0;
+    HIPCHK(rows_cost(ra, c->row_cost.p, c->total.p, nullptr, &tb1, c->stream));
+    HIPCHK(items_count(ra, 0, c->item_count.p, c->item_off.p, nullptr, &tb2, c->stream));
+    int st = ensure_temp(c, std::max(tb1, tb2));
+    if (st)
+        return st;
+    tb1 = tb2 = c->temp.cap;
+    HIPCHK(rows_cost(ra, c->row_cost.p, c->total.p, c->temp.p, &tb1, c->stream));
+    unsigned long long total = 0;
+    HIPCHK(hipMemcpyAsync(&total, c->total.p, sizeof(total), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->list_pairs = total;
+
+    long long split = 0;
+    if (queue) {
+        if (c->tuning.split_pairs > 0) {
+            split = c->tuning.split_pairs;
+        } else {
+            // about 8 items per persistent wave, within [2^14, 2^22] pairs
+            const long long waves = (long long)queue_groups(c) * kWavesPerGroup;
+            split = (long long)(total / (unsigned long long)std::max(1ll, 8 * waves));
+            split = std::max(split, 1ll << 14);
+            split = std::min(split, 1ll << 22);
+        }
+    }
+    HIPCHK(launch_fill_i32(c->item_count.p + nr, 1, 0, c->stream));
+    HIPCHK(items_count(ra, split, c->item_count.p, c->item_off.p, c->temp.p, &tb2, c->stream));
+    int nitems = 0;
+    HIPCHK(hipMemcpyAsync(&nitems, c->item_off.p + nr, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (!queue) {
+        HIPCHK(c->items_plain.reserve(std::max(nitems, 1)));
+        HIPCHK(items_emit(ra, 0, c->item_off.p, c->items_plain.p, nullptr, c->stream));
+        c->h_item_off_plain.resize(nr + 1);
+        HIPCHK(hipMemcpyAsync(c->h_item_off_plain.data(), c->item_off.p, sizeof(int) * (nr + 1),
+                              hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        c->nitems_plain = nitems;
+        return PP2P_OK;
+    }
+    HIPCHK(c->items_q_raw.reserve(std::max(nitems, 1)));
+    HIPCHK(c->items_q.reserve(std::max(nitems, 1)));
+    HIPCHK(c->cost.reserve(std::max(nitems, 1)));
+    HIPCHK(c->cost_sorted.reserve(std::max(nitems, 1)));
+    HIPCHK(c->idx.reserve(std::max(nitems, 1)));
+    HIPCHK(c->idx_sorted.reserve(std::max(nitems, 1)));
+    HIPCHK(items_emit(ra, split, c->item_off.p, c->items_q_raw.p, c->cost.p, c->stream));
+    size_t tb3 = 0;
+    HIPCHK(items_sort(c->cost.p, c->cost_sorted.p, c->idx.p, c->idx_sorted.p, c->items_q_raw.p, c->items_q.p, nitems,
+                      nullptr, &tb3, c->stream));
+    if ((st = ensure_temp(c, tb3)))
+        return st;
+    tb3 = c->temp.cap;
+    HIPCHK(items_sort(c->cost.p, c->cost_sorted.p, c->idx.p, c->idx_sorted.p, c->items_q_raw.p, c->items_q.p, nitems,
+                      c->temp.p, &tb3, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->nitems_q = nitems;
+    return PP2P_OK;
+}
+
+static int prepare_form(pp2p_ctx *c, bool queue)
+{
+    int st;
+    if (!c->csr_ready && (st = build_csr(c)))
+        return st;
+    if (queue && !c->queue_ready) {
+        if ((st = build_items(c, true)))
+            return st;
+        c->queue_ready = true;
+    }
+    if (!queue && !c->plain_ready) {
+        if ((st = build_items(c, false)))
+            return st;
+        c->plain_ready = true;
+    }
+    return PP2P_OK;
+}
+
+int pp2p_prepare(pp2p_ctx *c)
+{
+    int st = check_ctx(c);
+    if (st)
+        return st;
+    if (!c->have_particles)
+        return fail(PP2P_ERR_STATE, "set_particles before prepare");
+    HIPCHK(hipEventRecord(c->ev_p0, c->stream));
+    c->csr_ready = c->queue_ready = false;
+    if ((st = prepare_form(c, true)))
+        return st;
+    HIPCHK(hipEventRecord(c->ev_p1, c->stream));
+    HIPCHK(hipEventSynchronize(c->ev_p1));
+    HIPCHK(hipEventElapsedTime(&c->last_prepare_ms, c->ev_p0, c->ev_p1));
+    return PP2P_OK;
+}
+
+static ForceArgs force_args(pp2p_ctx *c, bool queue)
+{
+    ForceArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.items = queue ? c->items_q.p : c->items_plain.p;
+    a.nitems = queue ? c->nitems_q : c->nitems_plain;
+    a.item_base = 0;
+    a.counter = c->counter.p;
+    a.pos = c->pos.p;
+    a.ref = c->ref.p;
+    a.rng = c->rng.p;
+    a.src = c->src.p;
+    a.ilist = c->has_act ? c->ilist.p : nullptr;
+    a.acc = c->acc.p;
+    a.npart = c->npart;
+    a.box = c->params.box / c->law.rc;
+    a.out_scale = c->params.mass / (c->law.rc * c->law.rc);
+    a.table = c->table.p;
+    a.table_len = c->law.table_len;
+    a.cull = c->tuning.no_cull ? 0 : 1;
+    return a;
+}
+
+// enqueue the launches of one evaluation (no synchronisation; graph-safe)
+static int enqueue_compute(pp2p_ctx *c, int exec)
+{
+    const bool queue = exec == PP2P_EXEC_QUEUE;
+    ForceArgs a = force_args(c, queue);
+    const ForceVariant v = variant(c, queue);
+    if (a.nitems == 0)
+        return PP2P_OK;
+    if (queue) {
+        HIPCHK(launch_reset_counter(c->counter.p, c->stream));
+        const int groups = std::min(queue_groups(c), (a.nitems + kWavesPerGroup - 1) / kWavesPerGroup);
+        HIPCHK(launch_force(a, c->law.lc, v, groups, c->stream));
+    } else if (exec == PP2P_EXEC_GRID) {
+        HIPCHK(launch_force(a, c->law.lc, v, (a.nitems + kWavesPerGroup - 1) / kWavesPerGroup, c->stream));
+    } else {
+        for (int r = 0; r < c->nrows; ++r) {
+            const int b = c->h_item_off_plain[r], e = c->h_item_off_plain[r + 1];
+            if (e <= b)
+                continue;
+            a.item_base = b;
+            a.nitems = e;
+            HIPCHK(launch_force(a, c->law.lc, v, (e - b + kWavesPerGroup - 1) / kWavesPerGroup, c->stream));
+        }
+    }
+    return PP2P_OK;
+}
+
+int pp2p_compute(pp2p_ctx *c, int exec)
+{
+    int st = check_ctx(c);
+    if (st)
+        return st;
+    if (!c->have_particles)
+        return fail(PP2P_ERR_STATE, "set_particles before compute");
+    if (exec != PP2P_EXEC_QUEUE && exec != PP2P_EXEC_GRID && exec != PP2P_EXEC_PER_TARGET)
+        return fail(PP2P_ERR_ARG, "unknown execution form");
+    if ((st = prepare_form(c, exec == PP2P_EXEC_QUEUE)))
+        return st;
+    HIPCHK(hipEventRecord(c->ev_k0, c->stream));
+    if ((st = enqueue_compute(c, exec)))
+        return st;
+    HIPCHK(hipEventRecord(c->ev_k1, c->stream));
+    c->kernel_timed = true;
+    return PP2P_OK;
+}
+
+int pp2p_zero_acc(pp2p_ctx *c)
+{
+    int st = check_ctx(c);
+    if (st)
+        return st;
+    HIPCHK(launch_fill_f64(c->acc.p, 3 * c->npart, 0.0, c->stream));
+    return PP2P_OK;
+}
+
+int pp2p_run_graph(pp2p_ctx *c, int exec, int nsteps, int nreplay)
+{
+    int st = check_ctx(c);
+    if (st)
+        return st;
+    if (!c->have_particles)
+        return fail(PP2P_ERR_STATE, "set_particles before run_graph");
+    if (nsteps < 1 || nreplay < 1)
+        return fail(PP2P_ERR_ARG, "nsteps and nreplay must be >= 1");
+    if (exec != PP2P_EXEC_QUEUE && exec != PP2P_EXEC_GRID && exec != PP2P_EXEC_PER_TARGET)
+        return fail(PP2P_ERR_ARG, "unknown execution form");
+    if ((st = prepare_form(c, exec == PP2P_EXEC_QUEUE)))
+        return st;
+    // occupancy queries are not capturable: resolve them before capture
+    (void)queue_groups(c);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int cst = PP2P_OK;
+    for (int s = 0; s < nsteps && cst == PP2P_OK; ++s) {
+        if (launch_fill_f64(c->acc.p, 3 * c->npart, 0.0, c->stream) != hipSuccess)
+            cst = fail(PP2P_ERR_DEVICE, "fill launch failed under capture");
+        else
+            cst = enqueue_compute(c, exec);
+    }
+    hipError_t e = hipStreamEndCapture(c->stream, &graph);
+    if (cst)
+        return cst;
+    if (e != hipSuccess)
+        return fail(PP2P_ERR_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        return fail(PP2P_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+    HIPCHK(hipEventRecord(c->ev_k0, c->stream));
+    for (int r = 0; r < nreplay; ++r) {
+        e = hipGraphLaunch(gexec, c->stream);
+        if (e != hipSuccess)
+            break;
+    }
+    hipError_t e2 = hipEventRecord(c->ev_k1, c->stream);
+    hipError_t e3 = hipStreamSynchronize(c->stream);
+    (void)hipGraphExecDestroy(gexec);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess || e2 != hipSuccess || e3 != hipSuccess)
+        return fail(PP2P_ERR_DEVICE, "graph replay failed");
+    c->kernel_timed = true;
+    return PP2P_OK;
+}
+
+int pp2p_sync(pp2p_ctx *c)
+{
+    int st = check_ctx(c);
+    if (st)
+        return st;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return PP2P_OK;
+}
+
+int pp2p_get_acc_f64(pp2p_ctx *c, double *out)
+{
+    int st = check_ctx(c);
+    if (st)
+        return st;
+    if (!out && c->npart)
+        return fail(PP2P_ERR_ARG, "null output");
+    std::vector<double> planes((size_t)3 * c->npart);
+    if (c->npart)
+        HIPCHK(hipMemcpyAsync(planes.data(), c->acc.p, sizeof(double) * 3 * c->npart, hipMemcpyDeviceToHost,
+                              c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (long long i = 0; i < c->npart; ++i)
+        for (int d = 0; d < 3; ++d)
+            out[3 * i + d] = planes[(size_t)d * c->npart + i];
+    return PP2P_OK;
+}
+
+int pp2p_get_acc_f32_soa(pp2p_ctx *c, float *out)
+{
+    int st = check_ctx(c);
+    if (st)
+        return st;
+    if (!out && c->npart)
+        return fail(PP2P_ERR_ARG, "null output");
+    std::vector<double> planes((size_t)3 * c->npart);
+    if (c->npart)
+        HIPCHK(hipMemcpyAsync(planes.data(), c->acc.p, sizeof(double) * 3 * c->npart, hipMemcpyDeviceToHost,
+                              c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (size_t k = 0; k < planes.size(); ++k)
+        out[k] = (float)planes[k];
+    return PP2P_OK;
+}
+
+int pp2p_list_stats(pp2p_ctx *c, long long *ntask, long long *nrows, long long *nitems, long long *npairs)
+{
+    int st = check_ctx(c);
+    if (st)
+        return st;
+    if (!c->have_particles)
+        return fail(PP2P_ERR_STATE, "set_particles before list_stats");
+    if ((st = prepare_form(c, true)))
+        return st;
+    if (ntask) *ntask = c->ntask;
+    if (nrows) *nrows = c->nrows;
+    if (nitems) *nitems = c->nitems_q;
+    if (npairs) *npairs = (long long)c->list_pairs;
+    return PP2P_OK;
+}
+
+double pp2p_last_kernel_ms(pp2p_ctx *c)
+{
+    if (!c || !c->kernel_timed)
+        return -1.0;
+    if (hipEventSynchronize(c->ev_k1) != hipSuccess)
+        return -1.0;
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, c->ev_k0, c->ev_k1) != hipSuccess)
+        return -1.0;
+    return ms;
+}
+
+double pp2p_last_prepare_ms(pp2p_ctx *c) { return c ? c->last_prepare_ms : -1.0; }
+
+}  // extern "C"

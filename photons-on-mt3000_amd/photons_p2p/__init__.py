@@ -1,0 +1,272 @@
+"""Python binding of libphotons_p2p.so, the gfx950 short-range P2P library.
+
+The binding is a thin ctypes layer over include/photons_p2p.h: numpy arrays
+in, numpy arrays out.  All computation happens in the HIP library; when the
+library is missing or no GPU is visible the calls raise, they never fall back
+to a CPU path.
+
+Reference: the leaf-pair P2P path of PhotoNs, task_compute_p2p_gpu
+(src/gpu_entry.cu:32-209) over P2P_kernel / P2P_core
+(src/gpu_kernel.impl.h:50-157, :364-437).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+ROOT_DIR = os.path.dirname(PKG_DIR)
+LIB_DIR = os.path.join(ROOT_DIR, "lib")
+CORE_LIB = os.path.join(LIB_DIR, "libphotons_p2p.so")
+DROPIN_LIB = os.path.join(LIB_DIR, "libp2p_hip.so")
+HOST_LIB = os.path.join(LIB_DIR, "libphotons_host.so")
+
+# pp2p_status
+OK, ERR_ARG, ERR_DEVICE, ERR_ALLOC, ERR_STATE, ERR_NOGPU = range(6)
+SOFT_REF, SOFT_PLUMMER = 0, 1
+LAW_TABLE, LAW_POLY = 0, 1
+FRAME_LEAF, FRAME_ABS32 = 0, 1
+EXEC_QUEUE, EXEC_GRID, EXEC_PER_TARGET = 0, 1, 2
+
+UCUT_REF = 2.8125   # src/gpu_kernel.impl.h:86
+
+
+class P2PError(RuntimeError):
+    """A pp2p_* call returned a non-zero status."""
+
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"pp2p status {status}: {msg}")
+        self.status = status
+
+
+class _Params(ctypes.Structure):
+    _fields_ = [("softening", ctypes.c_int), ("law", ctypes.c_int), ("frame", ctypes.c_int),
+                ("reserved", ctypes.c_int), ("eps", ctypes.c_double), ("rs", ctypes.c_double),
+                ("ucut", ctypes.c_double), ("mass", ctypes.c_double), ("box", ctypes.c_double)]
+
+
+class _Tuning(ctypes.Structure):
+    _fields_ = [("waves_per_cu", ctypes.c_int), ("split_pairs", ctypes.c_int),
+                ("segs_per_octave", ctypes.c_int), ("no_cull", ctypes.c_int),
+                ("reserved", ctypes.c_int * 4)]
+
+
+# Every symbol include/photons_p2p.h declares.
+CORE_SYMBOLS = (
+    "pp2p_version", "pp2p_device_count", "pp2p_last_error", "pp2p_create", "pp2p_destroy",
+    "pp2p_set_params", "pp2p_set_tuning", "pp2p_get_tuning", "pp2p_set_particles_f64",
+    "pp2p_set_particles_i32", "pp2p_set_active", "pp2p_set_tasks", "pp2p_add_tasks",
+    "pp2p_prepare", "pp2p_compute", "pp2p_zero_acc", "pp2p_run_graph", "pp2p_sync",
+    "pp2p_get_acc_f64", "pp2p_get_acc_f32_soa", "pp2p_list_stats", "pp2p_last_kernel_ms",
+    "pp2p_last_prepare_ms",
+)
+
+_lib = None
+
+
+def load(path: str = CORE_LIB) -> ctypes.CDLL:
+    """Loads libphotons_p2p.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FileNotFoundError(
+            f"{path} not built: run `make -C {ROOT_DIR}` or __graft_entry__.build()")
+    lib = ctypes.CDLL(path)
+    vp, i32, i64, dbl = ctypes.c_void_p, ctypes.c_int, ctypes.c_longlong, ctypes.c_double
+    sig = {
+        "pp2p_version": (ctypes.c_char_p, []),
+        "pp2p_device_count": (i32, []),
+        "pp2p_last_error": (ctypes.c_char_p, []),
+        "pp2p_create": (vp, [i32, ctypes.POINTER(i32)]),
+        "pp2p_destroy": (None, [vp]),
+        "pp2p_set_params": (i32, [vp, ctypes.POINTER(_Params)]),
+        "pp2p_set_tuning": (i32, [vp, ctypes.POINTER(_Tuning)]),
+        "pp2p_get_tuning": (i32, [vp, ctypes.POINTER(_Tuning)]),
+        "pp2p_set_particles_f64": (i32, [vp, i64, vp, i32, vp, vp]),
+        "pp2p_set_particles_i32": (i32, [vp, i64, vp, ctypes.c_size_t, dbl, i32, vp, vp]),
+        "pp2p_set_active": (i32, [vp, vp, i32]),
+        "pp2p_set_tasks": (i32, [vp, i64, vp, vp]),
+        "pp2p_add_tasks": (i32, [vp, i64, vp, vp]),
+        "pp2p_prepare": (i32, [vp]),
+        "pp2p_compute": (i32, [vp, i32]),
+        "pp2p_zero_acc": (i32, [vp]),
+        "pp2p_run_graph": (i32, [vp, i32, i32, i32]),
+        "pp2p_sync": (i32, [vp]),
+        "pp2p_get_acc_f64": (i32, [vp, vp]),
+        "pp2p_get_acc_f32_soa": (i32, [vp, vp]),
+        "pp2p_list_stats": (i32, [vp] + [ctypes.POINTER(i64)] * 4),
+        "pp2p_last_kernel_ms": (dbl, [vp]),
+        "pp2p_last_prepare_ms": (dbl, [vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+@dataclass
+class ForceLaw:
+    """Force-law parameters (pp2p_params).
+
+    eps: softening length (SoftenScale, src/setparam.c:23); rs: split radius
+    (splitRadius, src/setparam.c:29); ucut: cutoff on u = r_eff / (2 rs)
+    (2.8125, src/gpu_kernel.impl.h:86); mass: particle mass (MASSPART);
+    box: periodic box for minimum-image offsets (0 = open boundaries).
+    """
+    eps: float
+    rs: float
+    mass: float = 1.0
+    ucut: float = UCUT_REF
+    softening: int = SOFT_REF
+    law: int = LAW_TABLE
+    frame: int = FRAME_LEAF
+    box: float = 0.0
+
+    @property
+    def rc(self) -> float:
+        return 2.0 * self.rs * self.ucut
+
+
+class Context:
+    """One device context of the P2P library (pp2p_ctx)."""
+
+    def __init__(self, device: int = 0):
+        self._lib = load()
+        st = ctypes.c_int(0)
+        self._h = self._lib.pp2p_create(device, ctypes.byref(st))
+        if not self._h:
+            raise P2PError(st.value, self._lib.pp2p_last_error().decode())
+        self.npart = 0
+        self.nleaf = 0
+
+    def _chk(self, status: int):
+        if status != OK:
+            raise P2PError(status, self._lib.pp2p_last_error().decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.pp2p_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- configuration -------------------------------------------------------
+    def set_law(self, law: ForceLaw):
+        p = _Params(law.softening, law.law, law.frame, 0, law.eps, law.rs, law.ucut, law.mass, law.box)
+        self._chk(self._lib.pp2p_set_params(self._h, ctypes.byref(p)))
+
+    def set_tuning(self, waves_per_cu: int = 0, split_pairs: int = 0, segs_per_octave: int = 0,
+                   no_cull: int = 0):
+        t = _Tuning(waves_per_cu, split_pairs, segs_per_octave, no_cull)
+        self._chk(self._lib.pp2p_set_tuning(self._h, ctypes.byref(t)))
+
+    def get_tuning(self) -> dict:
+        t = _Tuning()
+        self._chk(self._lib.pp2p_get_tuning(self._h, ctypes.byref(t)))
+        return {"waves_per_cu": t.waves_per_cu, "split_pairs": t.split_pairs,
+                "segs_per_octave": t.segs_per_octave, "no_cull": t.no_cull}
+
+    # -- data ----------------------------------------------------------------
+    def set_particles(self, pos: np.ndarray, leaf_ip: np.ndarray, leaf_np: np.ndarray):
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        if pos.ndim != 2 or pos.shape[1] != 3:
+            raise ValueError("pos must be (N, 3)")
+        ip = np.ascontiguousarray(leaf_ip, dtype=np.int32)
+        npl = np.ascontiguousarray(leaf_np, dtype=np.int32)
+        if ip.shape != npl.shape:
+            raise ValueError("leaf_ip and leaf_np differ in length")
+        self._chk(self._lib.pp2p_set_particles_f64(self._h, pos.shape[0], _ptr(pos), ip.size, _ptr(ip), _ptr(npl)))
+        self.npart, self.nleaf = pos.shape[0], ip.size
+
+    def set_particles_i32(self, posi: np.ndarray, int2pos: float, leaf_ip: np.ndarray, leaf_np: np.ndarray):
+        """posi: (N, 3) int32, or a structured array whose first 12 bytes are posi."""
+        posi = np.ascontiguousarray(posi)
+        stride = posi.strides[0]
+        n = posi.shape[0]
+        ip = np.ascontiguousarray(leaf_ip, dtype=np.int32)
+        npl = np.ascontiguousarray(leaf_np, dtype=np.int32)
+        self._chk(self._lib.pp2p_set_particles_i32(self._h, n, _ptr(posi), stride, float(int2pos), ip.size,
+                                                   _ptr(ip), _ptr(npl)))
+        self.npart, self.nleaf = n, ip.size
+
+    def set_active(self, act: np.ndarray | None, level: int = 0):
+        if act is None:
+            self._chk(self._lib.pp2p_set_active(self._h, None, 0))
+            return
+        a = np.ascontiguousarray(act, dtype=np.uint8)
+        if a.size != self.npart:
+            raise ValueError("act must have one entry per particle")
+        self._chk(self._lib.pp2p_set_active(self._h, _ptr(a), level))
+
+    def set_tasks(self, task_t: np.ndarray, task_s: np.ndarray):
+        tt = np.ascontiguousarray(task_t, dtype=np.int32)
+        ts = np.ascontiguousarray(task_s, dtype=np.int32)
+        if tt.shape != ts.shape:
+            raise ValueError("task_t and task_s differ in length")
+        self._chk(self._lib.pp2p_set_tasks(self._h, tt.size, _ptr(tt), _ptr(ts)))
+
+    def add_tasks(self, task_t: np.ndarray, task_s: np.ndarray):
+        tt = np.ascontiguousarray(task_t, dtype=np.int32)
+        ts = np.ascontiguousarray(task_s, dtype=np.int32)
+        if tt.shape != ts.shape:
+            raise ValueError("task_t and task_s differ in length")
+        self._chk(self._lib.pp2p_add_tasks(self._h, tt.size, _ptr(tt), _ptr(ts)))
+
+    # -- execution -----------------------------------------------------------
+    def prepare(self) -> float:
+        self._chk(self._lib.pp2p_prepare(self._h))
+        return self._lib.pp2p_last_prepare_ms(self._h)
+
+    def compute(self, exec_form: int = EXEC_QUEUE):
+        self._chk(self._lib.pp2p_compute(self._h, exec_form))
+
+    def zero_acc(self):
+        self._chk(self._lib.pp2p_zero_acc(self._h))
+
+    def run_graph(self, exec_form: int = EXEC_QUEUE, nsteps: int = 1, nreplay: int = 1):
+        self._chk(self._lib.pp2p_run_graph(self._h, exec_form, nsteps, nreplay))
+
+    def sync(self):
+        self._chk(self._lib.pp2p_sync(self._h))
+
+    def acc(self) -> np.ndarray:
+        out = np.empty((self.npart, 3), dtype=np.float64)
+        self._chk(self._lib.pp2p_get_acc_f64(self._h, _ptr(out)))
+        return out
+
+    def acc_f32_soa(self) -> np.ndarray:
+        out = np.empty((3, self.npart), dtype=np.float32)
+        self._chk(self._lib.pp2p_get_acc_f32_soa(self._h, _ptr(out)))
+        return out
+
+    def list_stats(self) -> dict:
+        v = [ctypes.c_longlong(0) for _ in range(4)]
+        self._chk(self._lib.pp2p_list_stats(self._h, *[ctypes.byref(x) for x in v]))
+        return {"ntask": v[0].value, "nrows": v[1].value, "nitems": v[2].value, "npairs": v[3].value}
+
+    def last_kernel_ms(self) -> float:
+        return self._lib.pp2p_last_kernel_ms(self._h)
+
+
+def device_count() -> int:
+    return load().pp2p_device_count()

@@ -1,0 +1,152 @@
+"""GPU parity of the HIP P2P kernels against the FP64 oracle.
+
+Each case builds a small seeded workload, evaluates it through the C-ABI
+library on cuda:0 and compares with oracle_p2p_f64 on the same lists and
+force law.  Tolerances: per-particle relative error |a - a64| / |a64|,
+p99 <= 1e-5 and max <= 1e-4 for the leaf-relative forms (the north-star
+gate), looser for the absolute-FP32 reference form.
+"""
+import numpy as np
+import pytest
+
+import photons_p2p as pp
+import pyoracle
+from photons_p2p import workloads
+
+pytestmark = pytest.mark.gpu
+
+P99_TOL = 1e-5
+MAX_TOL = 1e-4
+
+
+def _small_uneven(seed=11, softening=1, **kw):
+    return workloads.uneven_pairs(seed=seed, lattice=24, n_pool=2000, max_src=80, reach=4,
+                                  total_pairs=6000, softening=softening, **kw)
+
+
+def _oracle(w, act=None, level=0, box=None):
+    prm = pyoracle.params(w.eps, w.rs, mass=w.mass, softening=w.softening,
+                          box=w.box if box is None else box)
+    csr = pyoracle.build_csr(w.task_t, w.task_s, w.nleaf)
+    return pyoracle.p2p_f64(prm, w.pos, w.leaf_ip, w.leaf_np, csr, act=act, level=level)
+
+
+def _run(ctx, w, exec_form=pp.EXEC_QUEUE, law=pp.LAW_TABLE, frame=pp.FRAME_LEAF, act=None, level=0,
+         box=None, **tuning):
+    print(f"case: {w.name} N={w.npart} L={w.nleaf} T={w.task_t.size} exec={exec_form} law={law} "
+          f"frame={frame}", flush=True)
+    ctx.set_law(pp.ForceLaw(eps=w.eps, rs=w.rs, mass=w.mass, softening=w.softening, law=law, frame=frame,
+                            box=w.box if box is None else box))
+    ctx.set_tuning(**tuning)
+    ctx.set_particles(w.pos, w.leaf_ip, w.leaf_np)
+    if act is not None:
+        ctx.set_active(act, level)
+    ctx.set_tasks(w.task_t, w.task_s)
+    ctx.compute(exec_form)
+    return ctx.acc()
+
+
+@pytest.mark.parametrize("exec_form", [pp.EXEC_QUEUE, pp.EXEC_GRID, pp.EXEC_PER_TARGET])
+def test_uneven_plummer_periodic(gpu_ctx, exec_form):
+    w = _small_uneven()
+    ref = _oracle(w)
+    a = _run(gpu_ctx, w, exec_form)
+    med, p99, mx = pyoracle.rel_err(a, ref)
+    print(f"rel err median {med:.3e} p99 {p99:.3e} max {mx:.3e}")
+    assert p99 <= P99_TOL and mx <= MAX_TOL
+
+
+@pytest.mark.parametrize("law", [pp.LAW_TABLE, pp.LAW_POLY])
+def test_ref_softening_laws(gpu_ctx, law):
+    w = _small_uneven(seed=12, softening=0)
+    ref = _oracle(w)
+    a = _run(gpu_ctx, w, law=law)
+    med, p99, mx = pyoracle.rel_err(a, ref)
+    print(f"law {law}: rel err median {med:.3e} p99 {p99:.3e} max {mx:.3e}")
+    assert p99 <= P99_TOL and mx <= MAX_TOL
+
+
+def test_segments_16_and_no_cull(gpu_ctx):
+    w = _small_uneven(seed=13)
+    ref = _oracle(w)
+    a16 = _run(gpu_ctx, w, segs_per_octave=16)
+    a_nc = _run(gpu_ctx, w, no_cull=1)
+    for a in (a16, a_nc):
+        med, p99, mx = pyoracle.rel_err(a, ref)
+        assert p99 <= P99_TOL and mx <= MAX_TOL
+    gpu_ctx.set_tuning()
+
+
+def test_active_subset(gpu_ctx):
+    w = _small_uneven(seed=14)
+    g = np.random.default_rng(5)
+    act = (g.random(w.npart) < 0.1).astype(np.uint8) * 3
+    ref = _oracle(w, act=act, level=3)
+    a = _run(gpu_ctx, w, act=act, level=3)
+    sel = act == 3
+    assert np.all(a[~sel] == 0.0)
+    med, p99, mx = pyoracle.rel_err(a[sel], ref[sel])
+    print(f"active: rel err median {med:.3e} p99 {p99:.3e} max {mx:.3e}")
+    assert p99 <= P99_TOL and mx <= MAX_TOL
+
+
+def test_graph_replay_matches_eager(gpu_ctx):
+    w = _small_uneven(seed=15)
+    eager = _run(gpu_ctx, w)
+    for _ in range(3):
+        gpu_ctx.run_graph(pp.EXEC_QUEUE, nsteps=3, nreplay=2)
+        rep = gpu_ctx.acc()
+        np.testing.assert_allclose(rep, eager, rtol=1e-12, atol=0)
+        gpu_ctx.zero_acc()
+        gpu_ctx.compute(pp.EXEC_QUEUE)
+        gpu_ctx.sync()
+
+
+def test_abs32_frame_matches_reference_form(gpu_ctx):
+    """FP32 absolute coordinates: compare with the FP32 restatement of the
+    reference kernel on the same float32 positions (open boundaries)."""
+    w = _small_uneven(seed=16)
+    w.pos = w.pos.astype(np.float32).astype(np.float64)
+    a = _run(gpu_ctx, w, frame=pp.FRAME_ABS32, box=0.0)
+    ref64 = _oracle(w, box=0.0)
+    med, p99, mx = pyoracle.rel_err(a, ref64)
+    print(f"abs32 vs f64: median {med:.3e} p99 {p99:.3e} max {mx:.3e}")
+    assert p99 <= 1e-4
+
+
+def test_i32_positions(gpu_ctx):
+    """Reference integer positions (Body.posi) uploaded with a 40-byte stride."""
+    w = _small_uneven(seed=17)
+    int2pos = 100.0 / 2 ** 30
+    posi = np.round(w.pos / int2pos).astype(np.int32)
+    body = np.zeros(w.npart, dtype=[("posi", np.int32, 3), ("rest", np.uint8, 28)])
+    body["posi"] = posi
+    w.pos = posi.astype(np.float64) * int2pos
+    ref = _oracle(w)
+    ctx = gpu_ctx
+    ctx.set_law(pp.ForceLaw(eps=w.eps, rs=w.rs, softening=w.softening, box=w.box))
+    ctx.set_particles_i32(body, int2pos, w.leaf_ip, w.leaf_np)
+    ctx.set_tasks(w.task_t, w.task_s)
+    ctx.compute()
+    a = ctx.acc()
+    med, p99, mx = pyoracle.rel_err(a, ref)
+    assert p99 <= P99_TOL and mx <= MAX_TOL
+
+
+def test_streamed_lists_accumulate(gpu_ctx):
+    """Flushes of one list (add_tasks) equal one list; the accumulator sums
+    repeated computes like dev_part_acc across flush calls."""
+    w = _small_uneven(seed=18)
+    ref = _oracle(w)
+    ctx = gpu_ctx
+    ctx.set_law(pp.ForceLaw(eps=w.eps, rs=w.rs, softening=w.softening, box=w.box))
+    ctx.set_particles(w.pos, w.leaf_ip, w.leaf_np)
+    h = w.task_t.size // 3
+    ctx.set_tasks(w.task_t[:h], w.task_s[:h])
+    ctx.add_tasks(w.task_t[h:], w.task_s[h:])
+    ctx.compute()
+    a = ctx.acc()
+    assert pyoracle.rel_err(a, ref)[1] <= P99_TOL
+    ctx.compute()
+    a2 = ctx.acc()
+    np.testing.assert_allclose(a2, 2 * a, rtol=1e-12, atol=1e-300)
