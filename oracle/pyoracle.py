@@ -139,3 +139,49 @@ def rel_err(a, ref):
     if e.size == 0:
         return 0.0, 0.0, 0.0
     return float(np.median(e)), float(np.percentile(e, 99)), float(e.max())
+
+
+# --------------------------------------------------------------------------
+# Reference list producer (oracle/_ref/libreflist.so, built from the
+# reference's own src/fmm.c by oracle/ref.mk in the build container)
+
+REF_LIB = os.path.join(ORACLE_DIR, "_ref", "libreflist.so")
+
+
+def reflist_available() -> bool:
+    return os.path.exists(REF_LIB)
+
+
+def reflist_build(posi, tag, int2pos, maxleaf, open_angle, cutoff_radius, bdl, bdr) -> dict:
+    """Runs the reference's build_localtree + walk_task_p2p on the given
+    particles; returns perm, leaf_ip, leaf_np, task_t, task_s (0-based)."""
+    lib = ctypes.CDLL(REF_LIB)
+    vp = ctypes.c_void_p
+    ip_ = ctypes.POINTER(ctypes.c_int)
+    lib.reflist_build.restype = ctypes.c_int
+    lib.reflist_build.argtypes = [ctypes.c_longlong, vp, vp, ctypes.c_double, ctypes.c_int, ctypes.c_float,
+                                  ctypes.c_float, vp, vp, ctypes.POINTER(ip_), ctypes.POINTER(ctypes.c_int),
+                                  ctypes.POINTER(ip_), ctypes.POINTER(ip_), ctypes.POINTER(ctypes.c_longlong),
+                                  ctypes.POINTER(ip_), ctypes.POINTER(ip_)]
+    lib.reflist_free.argtypes = [vp]
+    posi = np.ascontiguousarray(posi, np.int32)
+    tag = np.ascontiguousarray(tag, np.int32)
+    bdl = np.ascontiguousarray(bdl, np.float32)
+    bdr = np.ascontiguousarray(bdr, np.float32)
+    perm, lip, lnp, tt, ts = ip_(), ip_(), ip_(), ip_(), ip_()
+    nleaf = ctypes.c_int(0)
+    ntask = ctypes.c_longlong(0)
+    rc = lib.reflist_build(posi.shape[0], _p(posi), _p(tag), int2pos, maxleaf, open_angle, cutoff_radius,
+                           _p(bdl), _p(bdr), ctypes.byref(perm), ctypes.byref(nleaf), ctypes.byref(lip),
+                           ctypes.byref(lnp), ctypes.byref(ntask), ctypes.byref(tt), ctypes.byref(ts))
+    if rc != 0:
+        raise RuntimeError("reflist_build failed")
+    n, nl, nt = posi.shape[0], nleaf.value, ntask.value
+    out = {"perm": np.ctypeslib.as_array(perm, (n,)).copy(),
+           "leaf_ip": np.ctypeslib.as_array(lip, (nl,)).copy(),
+           "leaf_np": np.ctypeslib.as_array(lnp, (nl,)).copy(),
+           "task_t": np.ctypeslib.as_array(tt, (nt,)).copy() if nt else np.zeros(0, np.int32),
+           "task_s": np.ctypeslib.as_array(ts, (nt,)).copy() if nt else np.zeros(0, np.int32)}
+    for p in (perm, lip, lnp, tt, ts):
+        lib.reflist_free(ctypes.cast(p, vp))
+    return out

@@ -1,0 +1,446 @@
+// Interaction-list producer: kd-tree over integer particle positions and the
+// symmetric dual-tree walk that emits (target leaf, source leaf) P2P pairs.
+//
+// This is the input side of the P2P path; it restates the reference's
+// single-threaded tree build and walk so that the lists are the ones the
+// reference hands to task_compute_p2p_gpu:
+//   mean-split partition       src/fmm.c:168-224 (bksort_inplace, INTXYZ)
+//   recursive build, numbering src/fmm.c:228-270 (build_kdtree)
+//   node/leaf boxes            src/fmm.c:273-327 (center_kdtree)
+//   acceptance (MAC + cutoff)  src/fmm.c:411-479
+//   dual walk                  src/fmm.c:656-792 (walk_task_p2p)
+//   active targets             src/fmm.c:1226-1236 (leaf holds a tag==1 particle)
+// Geometry (split, widths, centres, acceptance) is evaluated in FP32 exactly
+// as the reference's cpuType arithmetic, the mean in FP64.  Particles are held
+// as SoA (three int32 coordinate planes + the original index) instead of the
+// 40-byte Body records; the partition moves them identically.
+//
+// Differences from the reference, by design: leaf and node ids start at 0
+// (the reference offsets them by first_leaf = NPART and first_node); the walk
+// runs its top levels in parallel (OpenMP tasks) and concatenates the per-task
+// outputs in depth-first order, which reproduces the sequential order exactly.
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include <omp.h>
+
+#include "photons_host.h"
+
+namespace {
+
+struct Node {
+    int son[2];      // >= 0: node id; < 0: leaf id as -(leaf + 1)
+    float split;
+    float width[3];
+    float center[3];
+};
+
+struct Leaf {
+    int ipart, npart;
+    int active;      // 0: holds a local particle (target), -1: ghosts only
+    float width[3];
+    float center[3];
+};
+
+struct Tree {
+    long long n = 0;
+    double int2pos = 0.0;
+    int maxleaf = 0;
+    std::vector<int> x[3];   // coordinate planes (permuted in place)
+    std::vector<int> idx;    // original particle index
+    std::vector<Node> node;
+    std::vector<Leaf> leaf;
+};
+
+inline double coord(const Tree &t, int d, long long i) { return (double)(t.int2pos * t.x[d][i]); }
+
+inline void swap_part(Tree &t, long long a, long long b)
+{
+    std::swap(t.x[0][a], t.x[0][b]);
+    std::swap(t.x[1][a], t.x[1][b]);
+    std::swap(t.x[2][a], t.x[2][b]);
+    std::swap(t.idx[a], t.idx[b]);
+}
+
+// Partition of [p, p + len) about the (sampled) mean of coordinate D; the
+// scan order, the sampling stride and the left count follow src/fmm.c:168-224.
+void mean_split(Tree &t, int D, long long p, long long len, long long np[2], double *split)
+{
+    np[0] = np[1] = 0;
+    if (len < 2) {
+        np[1] = len;
+        return;
+    }
+    if (len == 2) {
+        np[0] = np[1] = 1;
+        *split = 0.5 * (t.int2pos * (t.x[D][p] + t.x[D][p + 1]));
+        if (t.x[D][p] > t.x[D][p + 1])
+            swap_part(t, p, p + 1);
+        return;
+    }
+    long long mul = 1;
+    if (len > 100000)
+        mul = len / 100000;
+    double mean = 0.0;
+    for (long long n = 0; n < len; n += mul)
+        mean += coord(t, D, p + n);
+    mean /= (double)len;
+    mean *= (double)mul;
+
+    long long but = len - 1;
+    for (long long n = 0; n < but; ++n) {
+        if (coord(t, D, p + n) > mean) {
+            while (coord(t, D, p + but) > mean && but > n)
+                --but;
+            swap_part(t, p + n, p + but);
+        }
+    }
+    np[0] = but;
+    np[1] = len - but;
+    *split = mean;
+}
+
+// Depth-first build; ids in creation order (leaves and nodes separately).
+void build(Tree &t, int direct, long long ip, long long len, int inode)
+{
+    if (len == 0)
+        return;
+    long long np[2];
+    double split = 0.0;
+    mean_split(t, direct, ip, len, np, &split);
+    t.node[inode].split = (float)split;
+    const int nd = (direct + 1) % 3;
+    for (int n = 0; n < 2; ++n) {
+        if (np[n] <= t.maxleaf) {
+            Leaf l{};
+            l.ipart = (int)ip;
+            l.npart = (int)np[n];
+            t.node[inode].son[n] = -(int)t.leaf.size() - 1;
+            t.leaf.push_back(l);
+        } else {
+            const int id = (int)t.node.size();
+            t.node.push_back(Node{{-1, -1}, 0.f, {0, 0, 0}, {0, 0, 0}});
+            t.node[inode].son[n] = id;
+            build(t, nd, ip, np[n], id);
+        }
+        ip += np[n];
+    }
+}
+
+// Boxes from the split planes (src/fmm.c:273-327), FP32.
+void boxes(Tree &t, int direct, int inode, float left[3], float right[3])
+{
+    Node &b = t.node[inode];
+    for (int d = 0; d < 3; ++d) {
+        b.width[d] = right[d] - left[d];
+        b.center[d] = (float)(0.5 * (right[d] + left[d]));
+    }
+    const int nd = (direct + 1) % 3;
+    for (int n = 0; n < 2; ++n) {
+        const int s = b.son[n];
+        if (s < 0) {
+            Leaf &l = t.leaf[-s - 1];
+            for (int d = 0; d < 3; ++d) {
+                l.width[d] = b.width[d];
+                l.center[d] = b.center[d];
+            }
+            if (n == 0) {
+                l.width[direct] = b.split - left[direct];
+                l.center[direct] = (float)(0.5 * (left[direct] + b.split));
+            } else {
+                l.width[direct] = right[direct] - b.split;
+                l.center[direct] = (float)(0.5 * (right[direct] + b.split));
+            }
+        } else if (n == 0) {
+            const float keep = right[direct];
+            right[direct] = b.split;
+            boxes(t, nd, s, left, right);
+            right[direct] = keep;
+        } else {
+            const float keep = left[direct];
+            left[direct] = b.split;
+            boxes(t, nd, s, left, right);
+            left[direct] = keep;
+        }
+    }
+}
+
+// 1: well separated (multipole), 0: open, -1: beyond the cutoff (src/fmm.c:411-479)
+int accept(const float wi[3], const float wj[3], const float dist[3], float open_angle, float cutoff)
+{
+    float w[3], mn[3];
+    for (int d = 0; d < 3; ++d) {
+        w[d] = (float)((wi[d] + wj[d]) * 0.5);
+        mn[d] = dist[d] < 0.0f ? -dist[d] : dist[d];
+        mn[d] -= w[d];
+        if (mn[d] <= 0.0f)
+            mn[d] = 0.0f;
+    }
+    const float dd2 = dist[0] * dist[0] + dist[1] * dist[1] + dist[2] * dist[2];
+    if (mn[0] + mn[1] + mn[2] < 1.0e-15)
+        return 0;
+    const float dm2 = mn[0] * mn[0] + mn[1] * mn[1] + mn[2] * mn[2];
+    if (dm2 > cutoff * cutoff)
+        return -1;
+    float wmax = w[0];
+    if (w[1] > wmax)
+        wmax = w[1];
+    if (w[2] > wmax)
+        wmax = w[2];
+    wmax = (float)std::fabs(wmax);
+    wmax *= 2;
+    return (wmax * wmax < open_angle * open_angle * dd2) ? 1 : 0;
+}
+
+struct Walker {
+    const Tree &t;
+    float open_angle, cutoff;
+    std::vector<int> *tt, *ts;
+
+    // a: >= 0 node id, < 0 leaf -(id+1)
+    void emit(int lt, int ls)
+    {
+        if (t.leaf[lt].active >= 0) {
+            tt->push_back(lt);
+            ts->push_back(ls);
+        }
+    }
+
+    void walk(int im, int jm)
+    {
+        if (im == jm) {
+            if (im < 0) {
+                emit(-im - 1, -jm - 1);
+                return;
+            }
+            const Node &b = t.node[im];
+            walk(b.son[0], b.son[0]);
+            walk(b.son[0], b.son[1]);
+            walk(b.son[1], b.son[0]);
+            walk(b.son[1], b.son[1]);
+            return;
+        }
+        if (im < 0 && jm < 0) {
+            emit(-im - 1, -jm - 1);
+            return;
+        }
+        float dist[3];
+        if (im < 0) {
+            const Leaf &a = t.leaf[-im - 1];
+            const Node &b = t.node[jm];
+            for (int d = 0; d < 3; ++d)
+                dist[d] = a.center[d] - b.center[d];
+            if (accept(a.width, b.width, dist, open_angle, cutoff) == 0) {
+                walk(im, b.son[0]);
+                walk(im, b.son[1]);
+            }
+            return;
+        }
+        if (jm < 0) {
+            const Node &a = t.node[im];
+            const Leaf &b = t.leaf[-jm - 1];
+            for (int d = 0; d < 3; ++d)
+                dist[d] = a.center[d] - b.center[d];
+            if (accept(b.width, a.width, dist, open_angle, cutoff) == 0) {
+                walk(a.son[0], jm);
+                walk(a.son[1], jm);
+            }
+            return;
+        }
+        const Node &a = t.node[im];
+        const Node &b = t.node[jm];
+        for (int d = 0; d < 3; ++d)
+            dist[d] = a.center[d] - b.center[d];
+        if (accept(a.width, b.width, dist, open_angle, cutoff) == 0) {
+            if (a.width[0] + a.width[1] + a.width[2] > b.width[0] + b.width[1] + b.width[2]) {
+                walk(a.son[0], jm);
+                walk(a.son[1], jm);
+            } else {
+                walk(im, b.son[0]);
+                walk(im, b.son[1]);
+            }
+        }
+    }
+};
+
+// Expands the walk breadth-first into independent (im, jm) frames in
+// depth-first order, until there are enough to run in parallel.
+void expand(const Tree &t, float open_angle, float cutoff, std::vector<std::pair<int, int>> &frames,
+            size_t target)
+{
+    // one level of expansion = replace each frame by its children frames in
+    // the order the recursive walk would visit them; leaf-leaf frames stay
+    for (int round = 0; round < 24 && frames.size() < target; ++round) {
+        std::vector<std::pair<int, int>> next;
+        bool grew = false;
+        for (auto f : frames) {
+            const int im = f.first, jm = f.second;
+            if (im < 0 && jm < 0) {
+                next.push_back(f);
+                continue;
+            }
+            if (im == jm) {
+                const Node &b = t.node[im];
+                next.push_back({b.son[0], b.son[0]});
+                next.push_back({b.son[0], b.son[1]});
+                next.push_back({b.son[1], b.son[0]});
+                next.push_back({b.son[1], b.son[1]});
+                grew = true;
+                continue;
+            }
+            float dist[3];
+            if (im < 0) {
+                const Leaf &a = t.leaf[-im - 1];
+                const Node &b = t.node[jm];
+                for (int d = 0; d < 3; ++d)
+                    dist[d] = a.center[d] - b.center[d];
+                if (accept(a.width, b.width, dist, open_angle, cutoff) == 0) {
+                    next.push_back({im, b.son[0]});
+                    next.push_back({im, b.son[1]});
+                    grew = true;
+                }
+                continue;
+            }
+            if (jm < 0) {
+                const Node &a = t.node[im];
+                const Leaf &b = t.leaf[-jm - 1];
+                for (int d = 0; d < 3; ++d)
+                    dist[d] = a.center[d] - b.center[d];
+                if (accept(b.width, a.width, dist, open_angle, cutoff) == 0) {
+                    next.push_back({a.son[0], jm});
+                    next.push_back({a.son[1], jm});
+                    grew = true;
+                }
+                continue;
+            }
+            const Node &a = t.node[im];
+            const Node &b = t.node[jm];
+            for (int d = 0; d < 3; ++d)
+                dist[d] = a.center[d] - b.center[d];
+            if (accept(a.width, b.width, dist, open_angle, cutoff) == 0) {
+                if (a.width[0] + a.width[1] + a.width[2] > b.width[0] + b.width[1] + b.width[2]) {
+                    next.push_back({a.son[0], jm});
+                    next.push_back({a.son[1], jm});
+                } else {
+                    next.push_back({im, b.son[0]});
+                    next.push_back({im, b.son[1]});
+                }
+                grew = true;
+            }
+        }
+        frames.swap(next);
+        if (!grew)
+            break;
+    }
+}
+
+}  // namespace
+
+struct phost_lists {
+    Tree tree;
+    std::vector<int> task_t, task_s;
+};
+
+extern "C" {
+
+phost_lists *phost_build(long long n, const int *posi, const int *tag, double int2pos, int maxleaf,
+                         float open_angle, float cutoff_radius, const float bdl[3], const float bdr[3],
+                         int nthreads)
+{
+    if (n <= 0 || !posi || !tag || maxleaf < 1 || !(int2pos > 0.0))
+        return nullptr;
+    if (nthreads <= 0)
+        nthreads = omp_get_max_threads();
+    phost_lists *h = new phost_lists();
+    Tree &t = h->tree;
+    t.n = n;
+    t.int2pos = int2pos;
+    t.maxleaf = maxleaf;
+    for (int d = 0; d < 3; ++d)
+        t.x[d].resize(n);
+    t.idx.resize(n);
+    for (long long i = 0; i < n; ++i) {
+        for (int d = 0; d < 3; ++d)
+            t.x[d][i] = posi[3 * i + d];
+        t.idx[i] = (int)i;
+    }
+    t.node.reserve(2 * (n / std::max(1, maxleaf / 2)) + 16);
+    t.leaf.reserve(2 * (n / std::max(1, maxleaf / 2)) + 16);
+    t.node.push_back(Node{{-1, -1}, 0.f, {0, 0, 0}, {0, 0, 0}});
+    build(t, 0, 0, n, 0);
+    float left[3] = {bdl[0], bdl[1], bdl[2]}, right[3] = {bdr[0], bdr[1], bdr[2]};
+    boxes(t, 0, 0, left, right);
+    // targets: leaves holding a local particle (src/fmm.c:1226-1236)
+    for (Leaf &l : t.leaf) {
+        l.active = -1;
+        for (int k = 0; k < l.npart; ++k)
+            if (tag[t.idx[l.ipart + k]] == 1) {
+                l.active = 0;
+                break;
+            }
+    }
+    // walk: expand the top into frames, walk each frame in parallel, then
+    // concatenate in frame order (= the sequential depth-first order)
+    std::vector<std::pair<int, int>> frames{{0, 0}};
+    expand(t, open_angle, cutoff_radius, frames, (size_t)nthreads * 64);
+    std::vector<std::vector<int>> ft(frames.size()), fs(frames.size());
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+    for (long long k = 0; k < (long long)frames.size(); ++k) {
+        Walker w{t, open_angle, cutoff_radius, &ft[k], &fs[k]};
+        w.walk(frames[k].first, frames[k].second);
+    }
+    size_t total = 0;
+    for (auto &v : ft)
+        total += v.size();
+    h->task_t.reserve(total);
+    h->task_s.reserve(total);
+    for (size_t k = 0; k < frames.size(); ++k) {
+        h->task_t.insert(h->task_t.end(), ft[k].begin(), ft[k].end());
+        h->task_s.insert(h->task_s.end(), fs[k].begin(), fs[k].end());
+    }
+    return h;
+}
+
+long long phost_count(const phost_lists *h, int what)
+{
+    if (!h)
+        return -1;
+    switch (what) {
+    case PHOST_NPART: return h->tree.n;
+    case PHOST_NLEAF: return (long long)h->tree.leaf.size();
+    case PHOST_NNODE: return (long long)h->tree.node.size();
+    case PHOST_NTASK: return (long long)h->task_t.size();
+    default: return -1;
+    }
+}
+
+void phost_get_perm(const phost_lists *h, int *perm)
+{
+    std::memcpy(perm, h->tree.idx.data(), sizeof(int) * h->tree.n);
+}
+
+void phost_get_leaves(const phost_lists *h, int *ip, int *np, int *active, float *center, float *width)
+{
+    const auto &L = h->tree.leaf;
+    for (size_t l = 0; l < L.size(); ++l) {
+        if (ip) ip[l] = L[l].ipart;
+        if (np) np[l] = L[l].npart;
+        if (active) active[l] = L[l].active;
+        for (int d = 0; d < 3; ++d) {
+            if (center) center[3 * l + d] = L[l].center[d];
+            if (width) width[3 * l + d] = L[l].width[d];
+        }
+    }
+}
+
+void phost_get_tasks(const phost_lists *h, int *task_t, int *task_s)
+{
+    std::memcpy(task_t, h->task_t.data(), sizeof(int) * h->task_t.size());
+    std::memcpy(task_s, h->task_s.data(), sizeof(int) * h->task_s.size());
+}
+
+void phost_free(phost_lists *h) { delete h; }
+
+}  // extern "C"

@@ -1,0 +1,32 @@
+/* Host-side interaction-list producer (kd-tree + dual walk), C-ABI of
+ * libphotons_host.so.  Restates src/fmm.c:168-327, :411-479, :656-792 and the
+ * target marking of :1226-1236; see kdtree.cpp. */
+#ifndef PHOTONS_HOST_H
+#define PHOTONS_HOST_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { PHOST_NPART = 0, PHOST_NLEAF = 1, PHOST_NNODE = 2, PHOST_NTASK = 3 };
+
+typedef struct phost_lists phost_lists;
+
+/* posi: n x 3 int32 grid positions (position = int2pos * posi); tag: 1 local
+ * particle, 0 ghost copy.  bdl/bdr: root box (particle_combine bounds).
+ * Returns NULL on invalid input. */
+phost_lists *phost_build(long long n, const int *posi, const int *tag, double int2pos, int maxleaf,
+                         float open_angle, float cutoff_radius, const float bdl[3], const float bdr[3],
+                         int nthreads);
+long long phost_count(const phost_lists *h, int what);
+/* perm[k] = original index of the particle at tree position k */
+void phost_get_perm(const phost_lists *h, int *perm);
+/* any output pointer may be NULL; center/width are nleaf x 3 */
+void phost_get_leaves(const phost_lists *h, int *ip, int *np, int *active, float *center, float *width);
+void phost_get_tasks(const phost_lists *h, int *task_t, int *task_s);
+void phost_free(phost_lists *h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -98,3 +98,154 @@ def uneven_pairs(seed: int = 20261205, lattice: int = 64, box: float = 100.0, n_
         eps = cell / 40.0 if softening == 1 else 0.03 * cell
     return Workload("uneven", pos, leaf_ip.astype(np.int32), leaf_np, task_t, task_s, box, rs, eps,
                     softening=softening, meta={"lattice": lattice, "n_pool": n_pool, "total_pairs": total_pairs})
+
+
+# --------------------------------------------------------------------------
+# Reference integer grid, ghost images and the list producer
+
+BITWIDTH = 1 << 30     # inc/photoNs.h:237
+
+
+def int_grid(box: float, cutoff_radius: float) -> float:
+    """INT2POS of the reference: (BOXSIZE + 1.1 cutoffRadius) / 2^30 (src/setparam.c:32-35)."""
+    return (box + cutoff_radius * 1.1) / BITWIDTH
+
+
+def to_grid(pos: np.ndarray, box: float, int2pos: float) -> tuple[np.ndarray, int]:
+    """Periodic int32 grid positions of local particles and the box length in
+    grid units (the reference shifts ghost copies by an integer box,
+    src/boundary.c:647-687)."""
+    box_i = int(round(box / int2pos))
+    posi = np.floor(pos / int2pos).astype(np.int64) % box_i
+    return posi.astype(np.int32), box_i
+
+
+def ghost_images(posi: np.ndarray, box_i: int, width_i: int) -> tuple[np.ndarray, np.ndarray]:
+    """Periodic copies of local particles lying within width_i outside the
+    box [0, box_i): what a single domain covering the whole box imports as
+    boundary particles.  Returns (ghost posi, index of the local original)."""
+    out, src = [], []
+    p = posi.astype(np.int64)
+    near_lo = p < width_i
+    near_hi = p >= box_i - width_i
+    for sx in (-1, 0, 1):
+        for sy in (-1, 0, 1):
+            for sz in (-1, 0, 1):
+                if sx == sy == sz == 0:
+                    continue
+                m = np.ones(p.shape[0], bool)
+                for d, s in enumerate((sx, sy, sz)):
+                    if s == 1:
+                        m &= near_lo[:, d]      # shifted up by one box
+                    elif s == -1:
+                        m &= near_hi[:, d]
+                idx = np.nonzero(m)[0]
+                if idx.size == 0:
+                    continue
+                g = p[idx] + np.array([sx, sy, sz], np.int64) * box_i
+                keep = np.all((g > -width_i) & (g < box_i + width_i), axis=1)
+                out.append(g[keep])
+                src.append(idx[keep])
+    if not out:
+        return np.zeros((0, 3), np.int32), np.zeros(0, np.int64)
+    return np.concatenate(out).astype(np.int32), np.concatenate(src)
+
+
+def combine_bounds(posi_all: np.ndarray, int2pos: float, box: float) -> tuple[np.ndarray, np.ndarray]:
+    """Root box of the tree as particle_combine computes it (src/gravity.c:15-140):
+    FP32 min/max of float(posi * INT2POS) widened by 1e-6 BOXSIZE."""
+    f = (posi_all.astype(np.float64) * int2pos).astype(np.float32)
+    box32 = np.float32(box)
+    lo = np.minimum(f.min(axis=0), box32)
+    hi = np.maximum(f.max(axis=0), np.float32(0.0))
+    widen = 0.000001 * np.float64(box32)
+    bdl = (lo.astype(np.float64) - widen).astype(np.float32)
+    bdr = (hi.astype(np.float64) + widen).astype(np.float32)
+    return bdl, bdr
+
+
+def _host_lib():
+    import ctypes
+    import os
+    from . import HOST_LIB
+    if not os.path.exists(HOST_LIB):
+        raise FileNotFoundError(f"{HOST_LIB} not built")
+    lib = ctypes.CDLL(HOST_LIB)
+    vp = ctypes.c_void_p
+    lib.phost_build.restype = vp
+    lib.phost_build.argtypes = [ctypes.c_longlong, vp, vp, ctypes.c_double, ctypes.c_int, ctypes.c_float,
+                                ctypes.c_float, vp, vp, ctypes.c_int]
+    lib.phost_count.restype = ctypes.c_longlong
+    lib.phost_count.argtypes = [vp, ctypes.c_int]
+    lib.phost_get_perm.argtypes = [vp, vp]
+    lib.phost_get_leaves.argtypes = [vp, vp, vp, vp, vp, vp]
+    lib.phost_get_tasks.argtypes = [vp, vp, vp]
+    lib.phost_free.argtypes = [vp]
+    return lib
+
+
+def build_lists(posi: np.ndarray, tag: np.ndarray, int2pos: float, maxleaf: int, open_angle: float,
+                cutoff_radius: float, bdl: np.ndarray, bdr: np.ndarray, nthreads: int = 0) -> dict:
+    """kd-tree + dual walk (libphotons_host.so).  Returns perm (tree order ->
+    input index), leaf_ip, leaf_np, active, task_t, task_s (0-based leaf ids)."""
+    lib = _host_lib()
+    P = lambda a: a.ctypes.data_as(__import__("ctypes").c_void_p)  # noqa: E731
+    posi = np.ascontiguousarray(posi, np.int32)
+    tag = np.ascontiguousarray(tag, np.int32)
+    bdl = np.ascontiguousarray(bdl, np.float32)
+    bdr = np.ascontiguousarray(bdr, np.float32)
+    h = lib.phost_build(posi.shape[0], P(posi), P(tag), int2pos, maxleaf, open_angle, cutoff_radius,
+                        P(bdl), P(bdr), nthreads)
+    if not h:
+        raise ValueError("phost_build rejected its input")
+    try:
+        n = lib.phost_count(h, 0)
+        nl = lib.phost_count(h, 1)
+        nt = lib.phost_count(h, 3)
+        perm = np.empty(n, np.int32)
+        ip = np.empty(nl, np.int32)
+        npl = np.empty(nl, np.int32)
+        act = np.empty(nl, np.int32)
+        tt = np.empty(nt, np.int32)
+        ts = np.empty(nt, np.int32)
+        lib.phost_get_perm(h, P(perm))
+        lib.phost_get_leaves(h, P(ip), P(npl), P(act), None, None)
+        lib.phost_get_tasks(h, P(tt), P(ts))
+    finally:
+        lib.phost_free(h)
+    return {"perm": perm, "leaf_ip": ip, "leaf_np": npl, "active": act, "task_t": tt, "task_s": ts}
+
+
+def tree_workload(name: str, pos: np.ndarray, box: float, mesh: int, maxleaf: int, open_angle: float,
+                  eps: float, softening: int, periodic_ghosts: bool = True, nthreads: int = 0,
+                  meta: dict | None = None) -> Workload:
+    """Local particles in [0, box) -> reference integer grid, ghost copies
+    within cutoffRadius, tree + walk lists.  The returned positions are the
+    grid positions (posi * INT2POS, FP64) in tree order, ghosts included;
+    meta['tag'] marks local particles (1) and ghosts (0), meta['local_index']
+    maps each row back to the input particle."""
+    cell = box / mesh
+    rs = 1.25 * cell                                   # src/setparam.c:29
+    cutoff_radius = 4.5 * rs                           # src/setparam.c:30
+    int2pos = int_grid(box, cutoff_radius)
+    posi, box_i = to_grid(pos, box, int2pos)
+    if periodic_ghosts:
+        width_i = int(np.ceil(np.float32(cutoff_radius) / int2pos))
+        gpos, gsrc = ghost_images(posi, box_i, width_i)
+    else:
+        gpos, gsrc = np.zeros((0, 3), np.int32), np.zeros(0, np.int64)
+    allp = np.concatenate([posi, gpos])
+    tag = np.concatenate([np.ones(posi.shape[0], np.int32), np.zeros(gpos.shape[0], np.int32)])
+    local_index = np.concatenate([np.arange(posi.shape[0]), gsrc])
+    bdl, bdr = combine_bounds(allp, int2pos, box)
+    lists = build_lists(allp, tag, int2pos, maxleaf, np.float32(open_angle), np.float32(cutoff_radius),
+                        bdl, bdr, nthreads)
+    perm = lists["perm"]
+    m = dict(meta or {})
+    m.update({"int2pos": int2pos, "box_i": box_i, "posi": allp[perm], "tag": tag[perm],
+              "local_index": local_index[perm], "nlocal": posi.shape[0], "maxleaf": maxleaf,
+              "open_angle": open_angle, "cutoff_radius": cutoff_radius, "mesh": mesh,
+              "bdl": bdl, "bdr": bdr, "active_leaf": lists["active"]})
+    w = Workload(name, allp[perm].astype(np.float64) * int2pos, lists["leaf_ip"], lists["leaf_np"],
+                 lists["task_t"], lists["task_s"], 0.0, rs, eps, softening=softening, meta=m)
+    return w
