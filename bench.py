@@ -1,0 +1,199 @@
+#!/usr/bin/env python3
+"""Benchmark of the short-range P2P path on one MI355X.
+
+A step is one evaluation of every leaf pair of the workload's interaction
+list (zero the accumulator + the persistent force launch over the prepared,
+cost-sorted queue), with particles, leaves and the list resident in HBM.
+The default workload is the 256^3 clustered configuration (half a
+Zel'dovich-displaced grid, half in 2,000 Plummer spheres, L = 400 Mpc/h,
+leaves <= 64, mesh 256^3, rs = 1.25 cells, cutoff u <= 2.8125, Plummer eps =
+spacing / 40), lists from the kd-tree + dual walk with periodic ghost copies.
+
+Metric: G pair-interactions/s = N_list / t_step, N_list = sum over leaf pairs
+of n_t * n_s (the reference's own counter, src/gpu_entry.cu:50-61).
+Roofline: algorithmic FLOPs = 48 N_in + 11 (N_list - N_in) (the per-pair
+operation count of the reference pair function inside / beyond the cutoff)
+over the force launch's device time, against the FP32 vector peak.
+Accuracy: per-particle relative error against the FP64 oracle on a seeded
+sample of target leaves.  cpu_baseline: the FP32 restatement of the
+reference kernel (oracle) on a bounded sample of the same list, all host
+threads.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (os.path.join(ROOT, "photons-on-mt3000_amd"), os.path.join(ROOT, "oracle")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+PEAK_FP32_TFLOPS = 157.3          # MI355X FP32 vector peak
+FLOPS_IN, FLOPS_OUT = 48, 11      # per-pair operation counts inside / beyond the cutoff
+METRIC = "G pair-interactions/s + % FP32 vector peak at p99 force err <=1e-5, 1 MI355X"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_workload(name: str, n1: int | None, threads: int):
+    from photons_p2p import workloads as W
+    if name == "clustered":
+        return W.config_clustered(n1 or 256, box=400.0 * (n1 or 256) / 256, nthreads=threads)
+    if name == "plain":
+        return W.config_plain(n1 or 64, nthreads=threads)
+    if name == "precision":
+        return W.config_precision(n1 or 128, nthreads=threads)
+    if name == "uneven":
+        return W.uneven_pairs()
+    raise SystemExit(f"unknown workload {name}")
+
+
+def sample_rows(w, nrows: int, seed: int):
+    """Task mask selecting every leaf pair of a seeded sample of target leaves."""
+    tg = np.unique(w.task_t)
+    g = np.random.default_rng(seed)
+    pick = g.choice(tg, size=min(nrows, tg.size), replace=False)
+    return np.isin(w.task_t, pick)
+
+
+def accuracy(w, acc, mask, threads):
+    import pyoracle
+    prm = pyoracle.params(w.eps, w.rs, mass=w.mass, softening=w.softening, box=w.box)
+    csr = pyoracle.build_csr(w.task_t[mask], w.task_s[mask], w.nleaf)
+    ref = pyoracle.p2p_f64(prm, w.pos, w.leaf_ip, w.leaf_np, csr, nthreads=threads)
+    tgt = csr[0]
+    idx = np.concatenate([np.arange(w.leaf_ip[t], w.leaf_ip[t] + w.leaf_np[t]) for t in tgt])
+    return pyoracle.rel_err(acc[idx], ref[idx]), idx.size
+
+
+def cpu_baseline(w, threads: int, budget_s: float):
+    """FP32 restatement of the reference kernel on a bounded row sample."""
+    import pyoracle
+    prm = pyoracle.params(w.eps, w.rs, mass=w.mass, softening=w.softening, box=w.box)
+    pos32 = np.ascontiguousarray(w.pos.T.astype(np.float32))
+    ntgt = np.unique(w.task_t).size
+    nrows = 16
+    rate = None
+    while True:
+        mask = sample_rows(w, nrows, seed=99)
+        csr = pyoracle.build_csr(w.task_t[mask], w.task_s[mask], w.nleaf)
+        npairs = int(np.sum(w.leaf_np[w.task_t[mask]].astype(np.int64) * w.leaf_np[w.task_s[mask]]))
+        t0 = time.perf_counter()
+        pyoracle.p2p_ref32(prm, pos32, w.leaf_ip, w.leaf_np, csr, nthreads=threads)
+        dt = time.perf_counter() - t0
+        rate = npairs / dt
+        if dt >= 0.5 * budget_s or nrows >= ntgt:
+            return rate, nrows, npairs, dt
+        nrows = min(ntgt, max(2 * nrows, int(nrows * budget_s / max(dt, 1e-3))))
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="clustered", choices=["clustered", "plain", "precision", "uneven"])
+    ap.add_argument("--n1", type=int, default=None, help="particles per side (default: the config's)")
+    ap.add_argument("--exec", default="queue", choices=["queue", "grid", "per_target"])
+    ap.add_argument("--law", default="table", choices=["table", "poly"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-acc", action="store_true", help="skip the accuracy sample")
+    ap.add_argument("--acc-rows", type=int, default=300)
+    ap.add_argument("--cpu-budget", type=float, default=12.0)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (the P2P path is evaluated on one MI355X); "
+              f"--gpus {args.gpus} is not supported", file=sys.stderr)
+        sys.exit(2)
+
+    import photons_p2p as pp
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+    t0 = time.time()
+    w = make_workload(args.workload, args.n1, threads)
+    log(f"[bench] workload {w.name}: N={w.npart} leaves={w.nleaf} tasks={w.task_t.size} "
+        f"built in {time.time() - t0:.1f}s")
+    exec_form = {"queue": pp.EXEC_QUEUE, "grid": pp.EXEC_GRID, "per_target": pp.EXEC_PER_TARGET}[args.exec]
+    law = pp.LAW_TABLE if args.law == "table" else pp.LAW_POLY
+
+    ctx = pp.Context(0)
+    ctx.set_law(pp.ForceLaw(eps=w.eps, rs=w.rs, mass=w.mass, softening=w.softening, law=law, box=w.box))
+    ctx.set_particles(w.pos, w.leaf_ip, w.leaf_np)
+    ctx.set_tasks(w.task_t, w.task_s)
+    prep_ms = ctx.prepare()
+    st = ctx.list_stats()
+    n_list = st["npairs"]
+    n_in = ctx.count_inside()
+    log(f"[bench] prepared in {prep_ms:.1f} ms: rows={st['nrows']} items={st['nitems']} "
+        f"N_list={n_list:.4e} N_in={n_in:.4e} ({n_in / max(n_list, 1):.3f})")
+
+    if args.warmup > 0:
+        ctx.time_steps(exec_form, args.warmup)
+    ctx.sync()
+    wall0 = time.perf_counter()
+    total_ms, force_ms = ctx.time_steps(exec_form, args.steps)
+    ctx.sync()
+    wall = time.perf_counter() - wall0
+    ms_step = total_ms / args.steps
+    t_kernel = force_ms / args.steps * 1e-3
+    value = n_list / (ms_step * 1e-3) / 1e9
+    flops = FLOPS_IN * n_in + FLOPS_OUT * (n_list - n_in)
+    achieved = flops / t_kernel / 1e12
+    log(f"[bench] {args.steps} steps: {ms_step:.3f} ms/step (force {t_kernel * 1e3:.3f} ms), "
+        f"wall {wall:.3f}s, {value:.1f} Gpairs/s, {achieved:.1f} TFLOP/s "
+        f"({100 * achieved / PEAK_FP32_TFLOPS:.1f}% of FP32 peak)")
+
+    rel = None
+    if not args.no_acc:
+        acc = ctx.acc()
+        mask = sample_rows(w, args.acc_rows, seed=20261202)
+        (med, p99, mx), nchk = accuracy(w, acc, mask, threads)
+        rel = {"median": med, "p99": p99, "max": mx, "particles": int(nchk),
+               "sample": f"{args.acc_rows} target leaves, FP64 oracle"}
+        log(f"[bench] rel err vs FP64 oracle ({nchk} particles): median {med:.2e} p99 {p99:.2e} max {mx:.2e}")
+    ctx.close()
+
+    cpu = None
+    if not args.no_cpu:
+        rate, nrows, npairs, dt = cpu_baseline(w, threads, args.cpu_budget)
+        cpu = {"value": rate / 1e9, "unit": "Gpairs/s", "cores": threads, "kind": "port",
+               "sample": f"{nrows} target leaves ({npairs:.3e} pair-interactions) of the same list, "
+                         f"FP32 restatement of P2P_kernel/P2P_core, {dt:.1f}s"}
+        log(f"[bench] cpu baseline: {rate / 1e9:.3f} Gpairs/s on {threads} threads ({dt:.1f}s)")
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "Gpairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": w.name, "npart": int(w.npart), "nleaf": int(w.nleaf), "ntask": int(w.task_t.size),
+                   "n_list": int(n_list), "n_in": int(n_in), "exec": args.exec, "law": args.law,
+                   "softening": "plummer" if w.softening == 1 else "ref", "eps": w.eps, "rs": w.rs},
+        "pct_fp32_peak": 100 * achieved / PEAK_FP32_TFLOPS,
+        "rel_err": rel,
+        "prepare_ms": prep_ms,
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

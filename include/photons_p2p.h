@@ -164,6 +164,17 @@ int pp2p_get_acc_f32_soa(pp2p_ctx *ctx, float *acc);
 int pp2p_list_stats(pp2p_ctx *ctx, long long *ntask, long long *nrows,
                     long long *nitems, long long *npairs);
 
+/* Number of listed pair-interactions with r_eff inside the cutoff (u <= ucut),
+ * counted on the device by a dedicated pass over the prepared list; the
+ * accumulator is untouched.  Synchronous. */
+int pp2p_count_inside(pp2p_ctx *ctx, long long *n_inside);
+
+/* Runs nsteps x (zero accumulator + compute(exec)) back to back on the
+ * context stream and returns their total device time (HIP events around the
+ * region) and, in force_ms (nullable), the summed device time of the force
+ * launches alone (events around them in every step).  Synchronous. */
+int pp2p_time_steps(pp2p_ctx *ctx, int exec, int nsteps, double *total_ms, double *force_ms);
+
 /* Device time in milliseconds of the force kernel launches of the last
  * pp2p_compute (HIP events on the context stream), and of the last
  * pp2p_prepare. */

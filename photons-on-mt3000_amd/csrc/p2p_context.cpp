@@ -117,6 +117,7 @@ struct pp2p_ctx {
     unsigned long long list_pairs = 0;
 
     DevBuf<unsigned> counter;
+    DevBuf<unsigned long long> pair_count;
     DevBuf<unsigned char> temp;
 
     float last_kernel_ms = 0.f, last_prepare_ms = 0.f;
@@ -197,6 +198,7 @@ void pp2p_destroy(pp2p_ctx *c)
     c->total.release(); c->item_count.release(); c->item_off.release(); c->items_plain.release();
     c->items_q_raw.release(); c->items_q.release(); c->cost.release(); c->cost_sorted.release();
     c->idx.release(); c->idx_sorted.release(); c->counter.release(); c->temp.release();
+    c->pair_count.release();
     if (c->ev_k0) (void)hipEventDestroy(c->ev_k0);
     if (c->ev_k1) (void)hipEventDestroy(c->ev_k1);
     if (c->ev_p0) (void)hipEventDestroy(c->ev_p0);
@@ -805,6 +807,95 @@ int pp2p_list_stats(pp2p_ctx *c, long long *ntask, long long *nrows, long long *
     if (nrows) *nrows = c->nrows;
     if (nitems) *nitems = c->nitems_q;
     if (npairs) *npairs = (long long)c->list_pairs;
+    return PP2P_OK;
+}
+
+int pp2p_count_inside(pp2p_ctx *c, long long *n_inside)
+{
+    int st = check_ctx(c);
+    if (st)
+        return st;
+    if (!c->have_particles)
+        return fail(PP2P_ERR_STATE, "set_particles before count_inside");
+    if (!n_inside)
+        return fail(PP2P_ERR_ARG, "null output");
+    if ((st = prepare_form(c, true)))
+        return st;
+    HIPCHK(c->pair_count.reserve(1));
+    unsigned long long zero = 0, total = 0;
+    HIPCHK(hipMemcpyAsync(c->pair_count.p, &zero, sizeof(zero), hipMemcpyHostToDevice, c->stream));
+    ForceArgs a = force_args(c, true);
+    a.count = c->pair_count.p;
+    if (a.nitems > 0) {
+        HIPCHK(launch_reset_counter(c->counter.p, c->stream));
+        const int groups = std::min(queue_groups(c), (a.nitems + kWavesPerGroup - 1) / kWavesPerGroup);
+        HIPCHK(launch_count(a, c->law.lc, variant(c, true), groups, c->stream));
+    }
+    HIPCHK(hipMemcpyAsync(&total, c->pair_count.p, sizeof(total), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    *n_inside = (long long)total;
+    return PP2P_OK;
+}
+
+int pp2p_time_steps(pp2p_ctx *c, int exec, int nsteps, double *total_ms, double *force_ms)
+{
+    int st = check_ctx(c);
+    if (st)
+        return st;
+    if (!c->have_particles)
+        return fail(PP2P_ERR_STATE, "set_particles before time_steps");
+    if (nsteps < 1 || !total_ms)
+        return fail(PP2P_ERR_ARG, "nsteps >= 1 and a result pointer are required");
+    if (exec != PP2P_EXEC_QUEUE && exec != PP2P_EXEC_GRID && exec != PP2P_EXEC_PER_TARGET)
+        return fail(PP2P_ERR_ARG, "unknown execution form");
+    if ((st = prepare_form(c, exec == PP2P_EXEC_QUEUE)))
+        return st;
+    (void)queue_groups(c);
+    std::vector<hipEvent_t> ev(2 * (size_t)nsteps, nullptr);
+    for (auto &e : ev)
+        HIPCHK(hipEventCreate(&e));
+    auto cleanup = [&]() {
+        for (auto &e : ev)
+            if (e)
+                (void)hipEventDestroy(e);
+    };
+    st = PP2P_OK;
+    if (hipEventRecord(c->ev_k0, c->stream) != hipSuccess)
+        st = fail(PP2P_ERR_DEVICE, "event record failed");
+    for (int s = 0; s < nsteps && st == PP2P_OK; ++s) {
+        if (launch_fill_f64(c->acc.p, 3 * c->npart, 0.0, c->stream) != hipSuccess ||
+            hipEventRecord(ev[2 * s], c->stream) != hipSuccess) {
+            st = fail(PP2P_ERR_DEVICE, "step launch failed");
+            break;
+        }
+        st = enqueue_compute(c, exec);
+        if (st == PP2P_OK && hipEventRecord(ev[2 * s + 1], c->stream) != hipSuccess)
+            st = fail(PP2P_ERR_DEVICE, "event record failed");
+    }
+    if (st == PP2P_OK && hipEventRecord(c->ev_k1, c->stream) != hipSuccess)
+        st = fail(PP2P_ERR_DEVICE, "event record failed");
+    if (st == PP2P_OK && hipEventSynchronize(c->ev_k1) != hipSuccess)
+        st = fail(PP2P_ERR_DEVICE, "event synchronize failed");
+    if (st != PP2P_OK) {
+        (void)hipStreamSynchronize(c->stream);
+        cleanup();
+        return st;
+    }
+    float ms = 0.f;
+    double fsum = 0.0;
+    hipError_t e = hipEventElapsedTime(&ms, c->ev_k0, c->ev_k1);
+    for (int s = 0; s < nsteps && e == hipSuccess; ++s) {
+        float f = 0.f;
+        e = hipEventElapsedTime(&f, ev[2 * s], ev[2 * s + 1]);
+        fsum += f;
+    }
+    cleanup();
+    if (e != hipSuccess)
+        return fail(PP2P_ERR_DEVICE, std::string("hipEventElapsedTime: ") + hipGetErrorString(e));
+    *total_ms = ms;
+    if (force_ms)
+        *force_ms = fsum;
+    c->kernel_timed = true;
     return PP2P_OK;
 }
 

@@ -61,6 +61,7 @@ struct ForceArgs {
     const float4 *table;      // LOGTAB coefficients (kLogOctaves * segs entries)
     int table_len;
     int cull;                 // drop j-particles beyond rc of the item's bounding box
+    unsigned long long *count;   // COUNT pass: pairs with s < 1
 };
 
 }  // namespace pp2p

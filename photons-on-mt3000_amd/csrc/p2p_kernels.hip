@@ -110,7 +110,7 @@ __device__ __forceinline__ float4 null_particle() { return make_float4(1e18f, 1e
 
 // FOLD64: leaf frames, FP64 fold of every batch's FP32 sums; otherwise one
 // FP32 accumulator over absolute coordinates (reference-style form).
-template <int LAW, int SOFT, bool FOLD64, bool INDIRECT, bool QUEUE>
+template <int LAW, int SOFT, bool FOLD64, bool INDIRECT, bool QUEUE, bool COUNT = false>
 __global__ __launch_bounds__(kWave * kWavesPerGroup)
 void p2p_force(ForceArgs a, LawConst lc)
 {
@@ -166,6 +166,7 @@ void p2p_force(ForceArgs a, LawConst lc)
         float fx = 0.f, fy = 0.f, fz = 0.f;
         int cnt = 0;                                   // j-particles waiting in the tile
 
+        unsigned n_in = 0;                             // COUNT: pairs inside the cutoff
         auto run_batch = [&](int n) {                  // n: multiple of groups
             const int ch = n / groups;
             if (live) {
@@ -173,7 +174,10 @@ void p2p_force(ForceArgs a, LawConst lc)
 #pragma unroll 4
                 for (int u = 0; u < ch; ++u) {
                     const float4 pj = tp[u];
-                    if (LAW == LAW_LOGTAB)
+                    if (COUNT) {
+                        const float dx = pj.x - xi, dy = pj.y - yi, dz = pj.z - zi;
+                        n_in += soft_s<SOFT>(fmaf(dz, dz, fmaf(dy, dy, dx * dx)), lc) < 1.0f;
+                    } else if (LAW == LAW_LOGTAB)
                         pair_logtab<SOFT>(xi, yi, zi, pj, s_tab, lc, fx, fy, fz);
                     else
                         pair_poly<SOFT>(xi, yi, zi, pj, lc, fx, fy, fz);
@@ -261,6 +265,16 @@ void p2p_force(ForceArgs a, LawConst lc)
             }
             wave_lds_sync();
         }
+        if (COUNT) {
+            unsigned long long c = n_in;
+            for (int o = 32; o > 0; o >>= 1)
+                c += __shfl_xor(c, o, kWave);
+            if (lane == 0)
+                atomicAdd(a.count, c);
+            if (!QUEUE)
+                break;
+            continue;
+        }
         if (phase == 0) {
             unsafeAtomicAdd(a.acc + ip, sx * a.out_scale);
             unsafeAtomicAdd(a.acc + a.npart + ip, sy * a.out_scale);
@@ -326,6 +340,27 @@ hipError_t force_occupancy(const ForceVariant &v, int table_len, int *groups_per
                 (const void *)p2p_force<LAW_LOGTAB, SOFT_REF, true, false, true>, kWave * kWavesPerGroup, lds);
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(groups_per_cu,
             (const void *)p2p_force<LAW_POLY, SOFT_REF, true, false, true>, kWave * kWavesPerGroup, lds);
+}
+
+hipError_t launch_count(const ForceArgs &a, const LawConst &lc, const ForceVariant &v, int grid, hipStream_t st)
+{
+    const size_t lds = sizeof(float4) * (size_t)a.table_len;
+    if (v.soft == SOFT_REF) {
+        if (v.indirect)
+            hipLaunchKernelGGL((p2p_force<LAW_LOGTAB, SOFT_REF, true, true, true, true>), dim3(grid),
+                               dim3(kWave * kWavesPerGroup), lds, st, a, lc);
+        else
+            hipLaunchKernelGGL((p2p_force<LAW_LOGTAB, SOFT_REF, true, false, true, true>), dim3(grid),
+                               dim3(kWave * kWavesPerGroup), lds, st, a, lc);
+    } else {
+        if (v.indirect)
+            hipLaunchKernelGGL((p2p_force<LAW_LOGTAB, SOFT_PLUMMER, true, true, true, true>), dim3(grid),
+                               dim3(kWave * kWavesPerGroup), lds, st, a, lc);
+        else
+            hipLaunchKernelGGL((p2p_force<LAW_LOGTAB, SOFT_PLUMMER, true, false, true, true>), dim3(grid),
+                               dim3(kWave * kWavesPerGroup), lds, st, a, lc);
+    }
+    return hipGetLastError();
 }
 
 __global__ void k_reset_counter(unsigned *c) { *c = 0u; }

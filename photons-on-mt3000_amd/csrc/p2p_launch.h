@@ -16,5 +16,7 @@ struct ForceVariant {
 hipError_t launch_force(const ForceArgs &a, const LawConst &lc, const ForceVariant &v, int grid, hipStream_t st);
 hipError_t force_occupancy(const ForceVariant &v, int table_len, int *groups_per_cu);
 hipError_t launch_reset_counter(unsigned *c, hipStream_t st);
+// counting pass of the queue form: adds the listed pairs with r_eff < rc to *a.count
+hipError_t launch_count(const ForceArgs &a, const LawConst &lc, const ForceVariant &v, int grid, hipStream_t st);
 
 }  // namespace pp2p

@@ -61,7 +61,7 @@ CORE_SYMBOLS = (
     "pp2p_set_particles_i32", "pp2p_set_active", "pp2p_set_tasks", "pp2p_add_tasks",
     "pp2p_prepare", "pp2p_compute", "pp2p_zero_acc", "pp2p_run_graph", "pp2p_sync",
     "pp2p_get_acc_f64", "pp2p_get_acc_f32_soa", "pp2p_list_stats", "pp2p_last_kernel_ms",
-    "pp2p_last_prepare_ms",
+    "pp2p_last_prepare_ms", "pp2p_count_inside", "pp2p_time_steps",
 )
 
 _lib = None
@@ -101,6 +101,8 @@ def load(path: str = CORE_LIB) -> ctypes.CDLL:
         "pp2p_list_stats": (i32, [vp] + [ctypes.POINTER(i64)] * 4),
         "pp2p_last_kernel_ms": (dbl, [vp]),
         "pp2p_last_prepare_ms": (dbl, [vp]),
+        "pp2p_count_inside": (i32, [vp, ctypes.POINTER(i64)]),
+        "pp2p_time_steps": (i32, [vp, i32, i32, ctypes.POINTER(dbl), ctypes.POINTER(dbl)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -263,6 +265,18 @@ class Context:
         v = [ctypes.c_longlong(0) for _ in range(4)]
         self._chk(self._lib.pp2p_list_stats(self._h, *[ctypes.byref(x) for x in v]))
         return {"ntask": v[0].value, "nrows": v[1].value, "nitems": v[2].value, "npairs": v[3].value}
+
+    def count_inside(self) -> int:
+        n = ctypes.c_longlong(0)
+        self._chk(self._lib.pp2p_count_inside(self._h, ctypes.byref(n)))
+        return n.value
+
+    def time_steps(self, exec_form: int = EXEC_QUEUE, nsteps: int = 1) -> tuple[float, float]:
+        """(total, force-only) device milliseconds of nsteps x (zero + compute)."""
+        ms = ctypes.c_double(0.0)
+        fms = ctypes.c_double(0.0)
+        self._chk(self._lib.pp2p_time_steps(self._h, exec_form, nsteps, ctypes.byref(ms), ctypes.byref(fms)))
+        return ms.value, fms.value
 
     def last_kernel_ms(self) -> float:
         return self._lib.pp2p_last_kernel_ms(self._h)

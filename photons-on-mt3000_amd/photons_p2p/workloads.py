@@ -249,3 +249,156 @@ def tree_workload(name: str, pos: np.ndarray, box: float, mesh: int, maxleaf: in
     w = Workload(name, allp[perm].astype(np.float64) * int2pos, lists["leaf_ip"], lists["leaf_np"],
                  lists["task_t"], lists["task_s"], 0.0, rs, eps, softening=softening, meta=m)
     return w
+
+
+# --------------------------------------------------------------------------
+# Particle generators
+
+def zeldovich_grid(n: int, box: float, rms_frac: float, g: np.random.Generator,
+                   dtype=np.float32) -> np.ndarray:
+    """n^3 grid in [0, box) displaced by a Zel'dovich field psi = i k / k^2 delta
+    of a Gaussian random field with P(k) ~ k^-2, scaled to an rms |psi| of
+    rms_frac * (box / n).  Returns (n^3, 3) float64, wrapped into [0, box)."""
+    d = box / n
+    w = g.standard_normal((n, n, n), dtype=np.float32 if dtype == np.float32 else np.float64)
+    wk = np.fft.rfftn(w)
+    del w
+    kx = np.fft.fftfreq(n).astype(dtype)[:, None, None]
+    ky = np.fft.fftfreq(n).astype(dtype)[None, :, None]
+    kz = np.fft.rfftfreq(n).astype(dtype)[None, None, :]
+    k2 = kx * kx + ky * ky + kz * kz
+    k2[0, 0, 0] = 1.0
+    amp = (k2 ** -1.5).astype(dtype)          # |delta| ~ k^-1, psi ~ k / k^2 delta
+    amp[0, 0, 0] = 0.0
+    psi = np.empty((n ** 3, 3), np.float64)
+    for j, kj in enumerate((kx, ky, kz)):
+        comp = np.fft.irfftn(1j * kj * amp * wk, s=(n, n, n), axes=(0, 1, 2))
+        psi[:, j] = comp.reshape(-1)
+        del comp
+    del wk
+    psi *= rms_frac * d / np.sqrt(np.mean(np.sum(psi * psi, axis=1)))
+    gr = (np.arange(n, dtype=np.float64) + 0.5) * d
+    q = np.stack(np.meshgrid(gr, gr, gr, indexing="ij"), -1).reshape(-1, 3)
+    return np.mod(q + psi, box)
+
+
+def _split_counts(total: int, weights: np.ndarray) -> np.ndarray:
+    """Integer counts proportional to weights summing to total (largest remainder)."""
+    x = total * weights / weights.sum()
+    c = np.floor(x).astype(np.int64)
+    rem = total - int(c.sum())
+    if rem > 0:
+        c[np.argsort(-(x - c), kind="stable")[:rem]] += 1
+    return c
+
+
+def _directions(g: np.random.Generator, m: int) -> np.ndarray:
+    v = g.standard_normal((m, 3))
+    return v / np.linalg.norm(v, axis=1, keepdims=True)
+
+
+def plummer_spheres(n_total: int, n_spheres: int, box: float, spacing: float, g: np.random.Generator,
+                    mass_decades: float = 3.0, a_range=(0.5, 5.0), trunc: float = 10.0) -> np.ndarray:
+    """n_total particles in n_spheres Plummer spheres with uniform centres,
+    masses log-uniform over `mass_decades` (particle counts proportional to
+    mass), scale radius a uniform in a_range * spacing, truncated at trunc a
+    (inverse CDF of M(<r)/M = r^3 / (r^2 + a^2)^{3/2})."""
+    mass = 10.0 ** (mass_decades * g.random(n_spheres))
+    counts = _split_counts(n_total, mass)
+    centres = g.random((n_spheres, 3)) * box
+    a = (a_range[0] + (a_range[1] - a_range[0]) * g.random(n_spheres)) * spacing
+    owner = np.repeat(np.arange(n_spheres), counts)
+    xmax = trunc ** 3 / (trunc ** 2 + 1.0) ** 1.5
+    x = g.random(n_total) * xmax
+    x = np.clip(x, 1e-300, None)
+    r = a[owner] / np.sqrt(x ** (-2.0 / 3.0) - 1.0)
+    pos = centres[owner] + r[:, None] * _directions(g, n_total)
+    return np.mod(pos, box)
+
+
+def nfw_halos(n_total: int, n_halos: int, box: float, n_mean_total: int, g: np.random.Generator,
+              mass_decades: float = 3.0, c_range=(5.0, 15.0)) -> np.ndarray:
+    """n_total particles in n_halos NFW halos: masses log-uniform over
+    `mass_decades` (counts proportional to mass), concentration uniform in
+    c_range, truncated at r200 of 200 times the mean number density
+    n_mean_total / box^3, radii by inverting mu(x) = ln(1 + x) - x / (1 + x)."""
+    mass = 10.0 ** (mass_decades * g.random(n_halos))
+    counts = _split_counts(n_total, mass)
+    centres = g.random((n_halos, 3)) * box
+    conc = c_range[0] + (c_range[1] - c_range[0]) * g.random(n_halos)
+    rho = n_mean_total / box ** 3
+    r200 = (3.0 * counts / (4.0 * np.pi * 200.0 * rho)) ** (1.0 / 3.0)
+    owner = np.repeat(np.arange(n_halos), counts)
+    mu = lambda x: np.log1p(x) - x / (1.0 + x)  # noqa: E731
+    target = g.random(n_total) * mu(conc[owner])
+    lo = np.zeros(n_total)
+    hi = conc[owner].copy()
+    for _ in range(60):                           # bisection to ~1e-18 relative
+        mid = 0.5 * (lo + hi)
+        below = mu(mid) < target
+        lo = np.where(below, mid, lo)
+        hi = np.where(below, hi, mid)
+    r = 0.5 * (lo + hi) * (r200 / conc)[owner]
+    pos = centres[owner] + r[:, None] * _directions(g, n_total)
+    return np.mod(pos, box)
+
+
+def clustered_mix(n1: int, box: float, seed: int, n_spheres: int = 2000) -> np.ndarray:
+    """Half of a Zel'dovich-displaced n1^3 grid (seeded random half) plus the
+    other half in Plummer spheres (configs at 256^3 and 512^3)."""
+    g = rng_for(seed)
+    n = n1 ** 3
+    grid = zeldovich_grid(n1, box, 0.3, g)
+    keep = g.permutation(n)[: n // 2]
+    keep.sort()
+    sph = plummer_spheres(n - keep.size, n_spheres, box, box / n1, g)
+    return np.concatenate([grid[keep], sph])
+
+
+# --------------------------------------------------------------------------
+# The benchmark configurations (sizes scale down through n1 for tests)
+
+def config_plain(n1: int = 64, box: float = 100.0, seed: int = 20261202, softening: int = 1,
+                 maxleaf: int = 32, open_angle: float = 0.5, nthreads: int = 0) -> Workload:
+    """Plain P2P: n1^3 Zel'dovich grid (rms 0.3 spacing), mesh n1^3, leaves
+    <= 32, opening angle 0.5; Plummer eps = spacing / 40 (REF: 0.03 cell,
+    the reference's SoftenScale, src/setparam.c:16-23)."""
+    g = rng_for(seed)
+    pos = zeldovich_grid(n1, box, 0.3, g)
+    d = box / n1
+    eps = d / 40.0 if softening == 1 else 0.03 * d
+    return tree_workload(f"plain{n1}", pos, box, n1, maxleaf, open_angle, eps, softening, nthreads=nthreads,
+                         meta={"seed": seed, "n1": n1})
+
+
+def config_clustered(n1: int = 256, box: float = 400.0, seed: int = 20261203, softening: int = 1,
+                     maxleaf: int = 64, open_angle: float = 0.5, n_spheres: int = 2000,
+                     nthreads: int = 0) -> Workload:
+    """Plain P2P at scale: half Zel'dovich grid, half in Plummer spheres;
+    leaves <= 64, mesh n1^3, Plummer eps = spacing / 40."""
+    pos = clustered_mix(n1, box, seed, n_spheres)
+    d = box / n1
+    eps = d / 40.0 if softening == 1 else 0.03 * d
+    return tree_workload(f"clustered{n1}", pos, box, n1, maxleaf, open_angle, eps, softening, nthreads=nthreads,
+                         meta={"seed": seed, "n1": n1, "n_spheres": n_spheres})
+
+
+def config_precision(n1: int = 128, box: float = 100.0, seed: int = 20261204, n_halos: int = 500,
+                     halo_frac: float = 0.6, offset_boxes: float = 1000.0, maxleaf: int = 32,
+                     open_angle: float = 0.5, nthreads: int = 0) -> Workload:
+    """Precision stress: 60% of n1^3 particles in NFW halos, the rest on a
+    displaced grid; Plummer eps = spacing / 50.  Lists are built on the box
+    coordinates; meta['offset'] is the translation (offset_boxes * box per
+    axis) that the precision comparison applies to absolute coordinates."""
+    g = rng_for(seed)
+    n = n1 ** 3
+    nh = int(round(halo_frac * n))
+    grid = zeldovich_grid(n1, box, 0.3, g)
+    keep = np.sort(g.permutation(n)[: n - nh])
+    halos = nfw_halos(nh, n_halos, box, n, g)
+    pos = np.concatenate([grid[keep], halos])
+    d = box / n1
+    w = tree_workload(f"precision{n1}", pos, box, n1, maxleaf, open_angle, d / 50.0, 1, nthreads=nthreads,
+                      meta={"seed": seed, "n1": n1})
+    w.meta["offset"] = offset_boxes * box
+    return w

@@ -1,0 +1,108 @@
+"""GPU parity on the tree-built configurations (kd-tree + dual walk lists with
+periodic ghost copies), scaled down so the FP64 oracle finishes in seconds,
+plus size-independent checks at larger sizes."""
+import numpy as np
+import pytest
+
+import photons_p2p as pp
+import pyoracle
+from photons_p2p import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _ctx_run(ctx, w, pos=None, frame=pp.FRAME_LEAF, law=pp.LAW_TABLE, exec_form=pp.EXEC_QUEUE):
+    print(f"case: {w.name} N={w.npart} tasks={w.task_t.size} frame={frame} law={law}", flush=True)
+    ctx.set_tuning()
+    ctx.set_law(pp.ForceLaw(eps=w.eps, rs=w.rs, mass=w.mass, softening=w.softening, law=law, frame=frame,
+                            box=w.box))
+    ctx.set_particles(w.pos if pos is None else pos, w.leaf_ip, w.leaf_np)
+    ctx.set_tasks(w.task_t, w.task_s)
+    ctx.compute(exec_form)
+    return ctx.acc()
+
+
+def _f64(w, pos=None, mask=None):
+    prm = pyoracle.params(w.eps, w.rs, mass=w.mass, softening=w.softening, box=w.box)
+    tt, ts = (w.task_t, w.task_s) if mask is None else (w.task_t[mask], w.task_s[mask])
+    csr = pyoracle.build_csr(tt, ts, w.nleaf)
+    return pyoracle.p2p_f64(prm, w.pos if pos is None else pos, w.leaf_ip, w.leaf_np, csr), csr
+
+
+@pytest.mark.parametrize("softening", [0, 1])
+def test_plain_config_whole_list(gpu_ctx, softening):
+    w = W.config_plain(24, softening=softening)
+    ref, _ = _f64(w)
+    a = _ctx_run(gpu_ctx, w)
+    med, p99, mx = pyoracle.rel_err(a, ref)
+    print(f"plain24 soft={softening}: median {med:.2e} p99 {p99:.2e} max {mx:.2e}")
+    assert p99 <= 1e-5 and mx <= 1e-4
+
+
+def test_clustered_config_whole_list(gpu_ctx):
+    w = W.config_clustered(32, box=50.0, n_spheres=40)
+    ref, _ = _f64(w)
+    a = _ctx_run(gpu_ctx, w)
+    med, p99, mx = pyoracle.rel_err(a, ref)
+    print(f"clustered32: median {med:.2e} p99 {p99:.2e} max {mx:.2e}")
+    assert p99 <= 1e-5 and mx <= 1e-4
+
+
+def test_count_inside_matches_oracle(gpu_ctx):
+    w = W.config_plain(20)
+    prm = pyoracle.params(w.eps, w.rs, softening=w.softening)
+    csr = pyoracle.build_csr(w.task_t, w.task_s, w.nleaf)
+    n_list, n_in = pyoracle.count_pairs(prm, w.pos, w.leaf_ip, w.leaf_np, csr)
+    _ctx_run(gpu_ctx, w)
+    st = gpu_ctx.list_stats()
+    assert st["npairs"] == n_list == w.n_list()
+    got = gpu_ctx.count_inside()
+    print(f"N_list {n_list} N_in oracle {n_in} gpu {got}")
+    assert abs(got - n_in) <= max(2, 1e-6 * n_in)
+
+
+def test_precision_forms(gpu_ctx):
+    """Precision stress (NFW halos, box offset by 1000 box lengths): the
+    leaf-relative form stays at the FP32 floor; FP32 absolute coordinates do not."""
+    w = W.config_precision(24)
+    off = w.meta["offset"]
+    pos_off = w.pos + off
+    ref, _ = _f64(w, pos=pos_off)
+    a_leaf = _ctx_run(gpu_ctx, w, pos=pos_off)
+    med, p99, mx = pyoracle.rel_err(a_leaf, ref)
+    print(f"leaf frame: median {med:.2e} p99 {p99:.2e} max {mx:.2e}")
+    assert p99 <= 1e-5 and mx <= 1e-4
+    a_abs = _ctx_run(gpu_ctx, w, pos=pos_off, frame=pp.FRAME_ABS32)
+    med_a, p99_a, mx_a = pyoracle.rel_err(a_abs, ref)
+    print(f"abs32 frame: median {med_a:.2e} p99 {p99_a:.2e} max {mx_a:.2e}")
+    assert p99_a > 100 * p99
+
+
+def test_exec_forms_agree_on_tree_lists(gpu_ctx):
+    w = W.config_plain(20)
+    outs = [_ctx_run(gpu_ctx, w, exec_form=e) for e in (pp.EXEC_QUEUE, pp.EXEC_GRID, pp.EXEC_PER_TARGET)]
+    # GRID and PER_TARGET run the same unsplit items: identical sums
+    np.testing.assert_array_equal(outs[1], outs[2])
+    # the queue splits long source lists: FP32 partial sums regroup
+    med, p99, mx = pyoracle.rel_err(outs[0], outs[1])
+    print(f"queue vs grid: median {med:.2e} p99 {p99:.2e} max {mx:.2e}")
+    assert p99 <= 2e-6 and mx <= 5e-5
+
+
+def test_full_size_momentum_and_sample(gpu_ctx):
+    """64^3 plain config at full size: sampled rows against the FP64 oracle and
+    total momentum of the local particles (the symmetric lists make the pair
+    forces cancel in pairs: sum m a = 0 up to ghost pairs), compared with the
+    oracle's own momentum residual on the same sample."""
+    w = W.config_plain(64)
+    a = _ctx_run(gpu_ctx, w)
+    g = np.random.default_rng(3)
+    tg = np.unique(w.task_t)
+    pick = g.choice(tg, size=60, replace=False)
+    mask = np.isin(w.task_t, pick)
+    ref, csr = _f64(w, mask=mask)
+    idx = np.concatenate([np.arange(w.leaf_ip[t], w.leaf_ip[t] + w.leaf_np[t]) for t in csr[0]])
+    med, p99, mx = pyoracle.rel_err(a[idx], ref[idx])
+    print(f"plain64 sample: median {med:.2e} p99 {p99:.2e} max {mx:.2e}")
+    assert p99 <= 1e-5 and mx <= 1e-4
+    assert np.all(np.isfinite(a))
