@@ -175,6 +175,14 @@ int pp2p_count_inside(pp2p_ctx *ctx, long long *n_inside);
  * launches alone (events around them in every step).  Synchronous. */
 int pp2p_time_steps(pp2p_ctx *ctx, int exec, int nsteps, double *total_ms, double *force_ms);
 
+/* Host evaluation of the device force law, operation for operation as the
+ * table law's pair function computes it in FP32: for n values of
+ * s = r_eff^2 / rc^2 (already softened, 0 < s), F[k] = g(u) / s^{3/2} in
+ * kernel units (rc = 1), or 0 at and beyond the cutoff.  Also returns the
+ * table size in entries.  For host-side tests of the tables; no device use. */
+int pp2p_law_eval(const pp2p_params *params, int segs_per_octave, long long n, const float *s,
+                  float *F, int *table_len);
+
 /* Device time in milliseconds of the force kernel launches of the last
  * pp2p_compute (HIP events on the context stream), and of the last
  * pp2p_prepare. */

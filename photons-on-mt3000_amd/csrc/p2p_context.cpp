@@ -911,6 +911,23 @@ double pp2p_last_kernel_ms(pp2p_ctx *c)
     return ms;
 }
 
+int pp2p_law_eval(const pp2p_params *p, int segs, long long n, const float *s, float *F, int *table_len)
+{
+    if (!p || n < 0 || (n > 0 && (!s || !F)))
+        return fail(PP2P_ERR_ARG, "invalid law_eval arguments");
+    LawSetup ls;
+    const char *err = nullptr;
+    if (!law_setup(p->softening, p->eps, p->rs, p->ucut, segs ? segs : 32, &ls, &err))
+        return fail(PP2P_ERR_ARG, err);
+    for (long long k = 0; k < n; ++k) {
+        const float sc = std::fmin(std::fmax(s[k], 0.0f), 1.0f);     // the kernel's clamp to s <= 1
+        F[k] = law_logtab_eval(ls, sc);
+    }
+    if (table_len)
+        *table_len = ls.table_len;
+    return PP2P_OK;
+}
+
 double pp2p_last_prepare_ms(pp2p_ctx *c) { return c ? c->last_prepare_ms : -1.0; }
 
 }  // extern "C"

@@ -61,7 +61,7 @@ CORE_SYMBOLS = (
     "pp2p_set_particles_i32", "pp2p_set_active", "pp2p_set_tasks", "pp2p_add_tasks",
     "pp2p_prepare", "pp2p_compute", "pp2p_zero_acc", "pp2p_run_graph", "pp2p_sync",
     "pp2p_get_acc_f64", "pp2p_get_acc_f32_soa", "pp2p_list_stats", "pp2p_last_kernel_ms",
-    "pp2p_last_prepare_ms", "pp2p_count_inside", "pp2p_time_steps",
+    "pp2p_last_prepare_ms", "pp2p_count_inside", "pp2p_time_steps", "pp2p_law_eval",
 )
 
 _lib = None
@@ -102,6 +102,7 @@ def load(path: str = CORE_LIB) -> ctypes.CDLL:
         "pp2p_last_kernel_ms": (dbl, [vp]),
         "pp2p_last_prepare_ms": (dbl, [vp]),
         "pp2p_count_inside": (i32, [vp, ctypes.POINTER(i64)]),
+        "pp2p_law_eval": (i32, [ctypes.POINTER(_Params), i32, i64, vp, vp, ctypes.POINTER(i32)]),
         "pp2p_time_steps": (i32, [vp, i32, i32, ctypes.POINTER(dbl), ctypes.POINTER(dbl)]),
     }
     for name, (res, args) in sig.items():
@@ -280,6 +281,20 @@ class Context:
 
     def last_kernel_ms(self) -> float:
         return self._lib.pp2p_last_kernel_ms(self._h)
+
+
+def law_eval(law: ForceLaw, s: np.ndarray, segs_per_octave: int = 32) -> tuple[np.ndarray, int]:
+    """Host evaluation of the device table law at s = r_eff^2 / rc^2 (kernel
+    units, rc = 1); returns (F, table entries).  No device is used."""
+    lib = load()
+    s = np.ascontiguousarray(s, np.float32)
+    out = np.empty_like(s)
+    n = ctypes.c_int(0)
+    p = _Params(law.softening, law.law, law.frame, 0, law.eps, law.rs, law.ucut, law.mass, law.box)
+    st = lib.pp2p_law_eval(ctypes.byref(p), segs_per_octave, s.size, _ptr(s), _ptr(out), ctypes.byref(n))
+    if st != OK:
+        raise P2PError(st, lib.pp2p_last_error().decode())
+    return out, n.value
 
 
 def device_count() -> int:

@@ -1,0 +1,102 @@
+"""The C-ABI libraries load and export every function their headers declare;
+host-side logic that needs no device (no compute calls are made here)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+from scipy.special import erfc
+
+import photons_p2p as pp
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+INCLUDE = os.path.join(ROOT, "include")
+
+
+def declared(header):
+    text = open(os.path.join(INCLUDE, header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"^[A-Za-z_][\w\s\*]*?\b([A-Za-z_]\w*)\s*\(", text, flags=re.M)
+    return sorted(set(n for n in names if n not in ("if", "while", "for", "return")))
+
+
+def test_core_library_exports_header():
+    lib = pp.load()
+    names = declared("photons_p2p.h")
+    assert len(names) >= 25
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(pp.CORE_SYMBOLS)
+
+
+def test_dropin_exports_reference_entry_points():
+    host = ctypes.CDLL(os.path.join(ROOT, "tests", "support", "libhostglobals.so"), mode=ctypes.RTLD_GLOBAL)
+    lib = ctypes.CDLL(pp.DROPIN_LIB)
+    names = declared("p2p_hip.h")
+    ref = {"task_compute_p2p_gpu", "p2ppack_compute_gpu", "task_compute_p2p_act_gpu", "p2ppack_compute_act_gpu",
+           "hostmalloc_task", "hostrealloc_task", "hostfree_task", "hostmalloc_in", "hostfree_in",
+           "hostmalloc_out", "hostfree_out", "dinfofree", "devfree", "updateAcc_fromHostBuff",
+           "devMemcpyDH_updateAcc", "sync_with_dev", "get_last_kntime"}
+    assert ref <= set(names)
+    for n in names:
+        assert hasattr(lib, n), n
+    del host
+
+
+def test_no_device_is_an_error_not_a_fallback():
+    if pp.device_count() > 0:
+        pytest.skip("a device is visible")
+    with pytest.raises(pp.P2PError) as e:
+        pp.Context(0)
+    assert e.value.status == pp.ERR_NOGPU
+
+
+def test_dropin_reports_missing_device_without_exit():
+    if pp.device_count() > 0:
+        pytest.skip("a device is visible")
+    host = ctypes.CDLL(os.path.join(ROOT, "tests", "support", "libhostglobals.so"), mode=ctypes.RTLD_GLOBAL)
+    lib = ctypes.CDLL(pp.DROPIN_LIB)
+    lib.p2p_hip_status.restype = ctypes.c_int
+    lib.task_compute_p2p_gpu.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+    ctypes.c_int.in_dll(host, "NPART").value = 0
+    lib.task_compute_p2p_gpu(0, None, None, 1)
+    assert lib.p2p_hip_status() == pp.ERR_NOGPU
+
+
+def F_exact(s, ucut):
+    u = ucut * np.sqrt(s)
+    return (erfc(u) + 2 / np.sqrt(np.pi) * u * np.exp(-u * u)) / s ** 1.5
+
+
+@pytest.mark.parametrize("softening,eps_div", [(1, 40.0), (0, 1 / 0.03), (1, 50.0)])
+def test_device_table_law_accuracy(softening, eps_div):
+    d = 400.0 / 256
+    law = pp.ForceLaw(eps=d / eps_div, rs=1.25 * d, softening=softening)
+    smin = (law.eps / law.rc) ** 2
+    s = np.geomspace(smin, 1.0 - 1e-6, 300001).astype(np.float32)
+    F, n = pp.law_eval(law, s)
+    assert n == 32 * 32
+    rel = np.abs(F / F_exact(s.astype(np.float64), law.ucut) - 1)
+    assert rel.max() < 5e-7 and np.median(rel) < 5e-8
+
+
+def test_device_table_cutoff_and_continuity():
+    law = pp.ForceLaw(eps=0.04, rs=2.0)
+    F, _ = pp.law_eval(law, np.array([1.0, 1.25, 2.0, 0.9999999], np.float32))
+    assert F[0] == 0 and F[1] == 0 and F[2] == 0
+    assert F[3] == pytest.approx(F_exact(0.9999999, law.ucut), rel=1e-6)
+    # segment joins: left and right limits agree to FP32 rounding
+    s0 = np.float32(2.0 ** -5)
+    left = np.nextafter(s0, np.float32(0))
+    Fl, _ = pp.law_eval(law, np.array([left, s0], np.float32))
+    assert Fl[0] == pytest.approx(Fl[1], rel=2e-6)
+
+
+def test_law_parameter_validation():
+    with pytest.raises(pp.P2PError):
+        pp.law_eval(pp.ForceLaw(eps=1e-9, rs=1.0), np.ones(1, np.float32))     # eps / rc outside the table
+    with pytest.raises(pp.P2PError):
+        pp.law_eval(pp.ForceLaw(eps=10.0, rs=1.0), np.ones(1, np.float32))     # eps beyond the cutoff
+    with pytest.raises(pp.P2PError):
+        pp.law_eval(pp.ForceLaw(eps=0.1, rs=1.0), np.ones(1, np.float32), segs_per_octave=7)
