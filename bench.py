@@ -108,6 +108,15 @@ def main():
     ap.add_argument("--no-acc", action="store_true", help="skip the accuracy sample")
     ap.add_argument("--acc-rows", type=int, default=300)
     ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--waves-per-cu", type=int, default=0, help="persistent waves per CU (0: occupancy max)")
+    ap.add_argument("--split", type=int, default=0, help="queue item size in pair-interactions (0: auto)")
+    ap.add_argument("--segs", type=int, default=0, help="table segments per octave (16/32, 0: 32)")
+    ap.add_argument("--no-cull", action="store_true")
+    ap.add_argument("--graph", type=int, default=0,
+                    help="capture this many steps in one hipGraph and time its replays (0: eager steps)")
+    ap.add_argument("--compare-plain", action="store_true",
+                    help="also time the one-launch-per-target form the same way and report the ratio")
+    ap.add_argument("--chunk-model", type=int, default=0, help="i-chunking: 0 default, -1 chunks of 64, >0 cost")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -126,6 +135,8 @@ def main():
 
     ctx = pp.Context(0)
     ctx.set_law(pp.ForceLaw(eps=w.eps, rs=w.rs, mass=w.mass, softening=w.softening, law=law, box=w.box))
+    ctx.set_tuning(waves_per_cu=args.waves_per_cu, split_pairs=args.split, segs_per_octave=args.segs,
+                   no_cull=int(args.no_cull), chunk_model=args.chunk_model)
     ctx.set_particles(w.pos, w.leaf_ip, w.leaf_np)
     ctx.set_tasks(w.task_t, w.task_s)
     prep_ms = ctx.prepare()
@@ -135,15 +146,32 @@ def main():
     log(f"[bench] prepared in {prep_ms:.1f} ms: rows={st['nrows']} items={st['nitems']} "
         f"N_list={n_list:.4e} N_in={n_in:.4e} ({n_in / max(n_list, 1):.3f})")
 
-    if args.warmup > 0:
-        ctx.time_steps(exec_form, args.warmup)
-    ctx.sync()
-    wall0 = time.perf_counter()
-    total_ms, force_ms = ctx.time_steps(exec_form, args.steps)
-    ctx.sync()
-    wall = time.perf_counter() - wall0
-    ms_step = total_ms / args.steps
-    t_kernel = force_ms / args.steps * 1e-3
+    def timed(form):
+        """(ms per step, force ms per step, wall seconds) of `form`."""
+        if args.graph > 0:
+            reps = max(1, args.steps // args.graph)
+            ctx.run_graph(form, nsteps=args.graph, nreplay=1)          # warm: capture + one replay
+            w0 = time.perf_counter()
+            ctx.run_graph(form, nsteps=args.graph, nreplay=reps)
+            wl = time.perf_counter() - w0
+            ms = ctx.last_kernel_ms() / (reps * args.graph)
+            return ms, ms, wl
+        if args.warmup > 0:
+            ctx.time_steps(form, args.warmup)
+        ctx.sync()
+        w0 = time.perf_counter()
+        tot, frc = ctx.time_steps(form, args.steps)
+        ctx.sync()
+        return tot / args.steps, frc / args.steps, time.perf_counter() - w0
+
+    ms_step, force_step, wall = timed(exec_form)
+    t_kernel = force_step * 1e-3
+    plain = None
+    if args.compare_plain:
+        p_ms, p_force, p_wall = timed(pp.EXEC_PER_TARGET)
+        plain = {"ms_per_step": p_ms, "value": n_list / (p_ms * 1e-3) / 1e9, "speedup_of_queue": p_ms / ms_step,
+                 "form": "one launch per target leaf", "graph_steps": args.graph}
+        log(f"[bench] per-target form: {p_ms:.3f} ms/step; queue form is {p_ms / ms_step:.2f}x faster")
     value = n_list / (ms_step * 1e-3) / 1e9
     flops = FLOPS_IN * n_in + FLOPS_OUT * (n_list - n_in)
     achieved = flops / t_kernel / 1e12
@@ -151,6 +179,7 @@ def main():
         f"wall {wall:.3f}s, {value:.1f} Gpairs/s, {achieved:.1f} TFLOP/s "
         f"({100 * achieved / PEAK_FP32_TFLOPS:.1f}% of FP32 peak)")
 
+    ctx_tuning = ctx.get_tuning()
     rel = None
     if not args.no_acc:
         acc = ctx.acc()
@@ -182,12 +211,15 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic",
+        "tuning": ctx_tuning,
         "config": {"workload": w.name, "npart": int(w.npart), "nleaf": int(w.nleaf), "ntask": int(w.task_t.size),
                    "n_list": int(n_list), "n_in": int(n_in), "exec": args.exec, "law": args.law,
                    "softening": "plummer" if w.softening == 1 else "ref", "eps": w.eps, "rs": w.rs},
         "pct_fp32_peak": 100 * achieved / PEAK_FP32_TFLOPS,
         "rel_err": rel,
         "prepare_ms": prep_ms,
+        "graph_steps": args.graph,
+        "plain_form": plain,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None},
         "cpu_baseline": cpu,

@@ -87,7 +87,10 @@ typedef struct {
                              (0 = automatic: ~8 items per persistent wave) */
     int segs_per_octave;  /* table law: 16 or 32 segments per octave of s (0 = 32) */
     int no_cull;          /* 1 disables the per-item j-culling (for measurement) */
-    int reserved[4];
+    int chunk_model;      /* largest i-chunk of a target per work item: <= 0 the
+                             kernel maximum of 256 (64 lanes x 4 per lane), > 0
+                             at most this many (balanced chunks) */
+    int reserved[3];
 } pp2p_tuning;
 
 typedef struct pp2p_ctx pp2p_ctx;

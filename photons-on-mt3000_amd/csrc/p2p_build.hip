@@ -165,6 +165,7 @@ struct RowView {
     const int *row_tgt, *row_off, *src;
     const int2 *rng;
     const int *act_cnt, *act_off;   // null: all particles of the target
+    float stage_ops;                 // chunking model (< 0: 64-particle chunks)
 };
 
 __device__ __forceinline__ void row_i(const RowView &v, int tgt, int &ibeg, int &ni)
@@ -193,47 +194,88 @@ __global__ void k_row_cost(RowView v, int nrows, unsigned long long *__restrict_
     cost[r] = ns * (unsigned long long)ni;
 }
 
-// Source ranges of a row: greedy cut when the running pairs of one i-chunk
-// (min(ni, 64) i-particles) would exceed `split`; split <= 0: one range.
+// Shape of an item with c i-particles: G = 2^glog phases (lane groups that
+// split the sources), IPL i-particles per lane, 64/G * IPL >= c slots.  The
+// wave time per source particle is proportional to IPL / G; the smallest value
+// wins, ties go to fewer phases.  48 -> G 4 x IPL 3 (all lanes busy),
+// 24 -> 8 x 3, 64 -> 1 x 1, 1 -> 64 x 1.
+__device__ __forceinline__ void item_shape(int c, int &glog, int &ipl)
+{
+    glog = 0;
+    ipl = (c + kWave - 1) / kWave;
+    float best = (float)ipl;
+    for (int g = 1; g <= 6; ++g) {
+        const int lanes = kWave >> g;
+        const int p = (c + lanes - 1) / lanes;
+        if (p > kIplMax)
+            continue;
+        const float t = (float)p / (float)(1 << g);
+        if (t < best * 0.999f) {
+            best = t;
+            glog = g;
+            ipl = p;
+        }
+    }
+}
+
+// i-chunks of a target with ni particles: k = ceil(ni / 256) near-equal chunks
+// (chunk_model > 0 asks for chunks of at most that many particles instead)
+__device__ __forceinline__ int n_chunks(int ni, float max_chunk)
+{
+    int cap = kItemMaxI;
+    if (max_chunk > 0.f && (int)max_chunk < cap)
+        cap = max((int)max_chunk, 1);
+    return (ni + cap - 1) / cap;
+}
+
+__device__ __forceinline__ void chunk_of(int ni, int k, int q, int &beg, int &size)
+{
+    const int base = ni / k, rem = ni % k;
+    beg = q * base + min(q, rem);
+    size = base + (q < rem ? 1 : 0);
+}
+
+// Source ranges of a row: greedy cut when the running pairs of the largest
+// i-chunk would exceed `split`; split <= 0: one range.  Every (range, chunk)
+// pair is one item; its cost is the modelled lane-work IPL * 64 / G * sum n_s.
 template <bool EMIT>
 __device__ int row_ranges(const RowView &v, int r, long long split, int tgt, int ibeg, int ni,
                           Item *items, unsigned *cost, int out0)
 {
-    const int ichunks = (ni + kItemMaxI - 1) / kItemMaxI;
     if (ni == 0)
         return 0;
-    const long long wi = ni < kItemMaxI ? ni : kItemMaxI;
+    const int k = n_chunks(ni, v.stage_ops);
+    const long long wi = (ni + k - 1) / k;
     const int k0 = v.row_off[r], k1 = v.row_off[r + 1];
     int nr = 0, out = out0;
     int beg = k0;
-    long long acc = 0;
-    for (int k = k0; k < k1; ++k) {
-        const long long c = wi * (long long)v.rng[v.src[k]].y;
-        if (split > 0 && k > beg && acc + c > split) {
-            if (EMIT)
-                for (int q = 0; q < ichunks; ++q) {
-                    const int nq = min(kItemMaxI, ni - q * kItemMaxI);
-                    items[out] = Item{tgt, ibeg + q * kItemMaxI, nq, r, beg, k, 0, 0};
-                    if (cost)
-                        cost[out] = (unsigned)min(acc / wi * nq, 0xffffffffll);
-                    ++out;
-                }
-            ++nr;
-            beg = k;
-            acc = 0;
-        }
-        acc += c;
-    }
-    if (EMIT)
-        for (int q = 0; q < ichunks; ++q) {
-            const int nq = min(kItemMaxI, ni - q * kItemMaxI);
-            items[out] = Item{tgt, ibeg + q * kItemMaxI, nq, r, beg, k1, 0, 0};
+    long long nsum = 0;
+    auto emit = [&](int s_beg, int s_end, long long ns) {
+        for (int q = 0; q < k; ++q) {
+            int cb, cs, glog, ipl;
+            chunk_of(ni, k, q, cb, cs);
+            item_shape(cs, glog, ipl);
+            items[out] = Item{tgt, ibeg + cb, cs, r, s_beg, s_end, glog, ipl};
             if (cost)
-                cost[out] = (unsigned)min(acc / wi * nq, 0xffffffffll);
+                cost[out] = (unsigned)min(ns * (long long)((kWave * ipl) >> glog), 0xffffffffll);
             ++out;
         }
+    };
+    for (int kk = k0; kk < k1; ++kk) {
+        const long long ns = v.rng[v.src[kk]].y;
+        if (split > 0 && kk > beg && (nsum + ns) * wi > split) {
+            if (EMIT)
+                emit(beg, kk, nsum);
+            ++nr;
+            beg = kk;
+            nsum = 0;
+        }
+        nsum += ns;
+    }
+    if (EMIT)
+        emit(beg, k1, nsum);
     ++nr;
-    return nr * ichunks;
+    return nr * k;
 }
 
 __global__ void k_items_count(RowView v, int nrows, long long split, int *__restrict__ count)
@@ -364,7 +406,7 @@ hipError_t rows_cost(const RowArgs &ra, unsigned long long *cost, unsigned long 
     if (temp == nullptr)
         return rocprim::reduce(nullptr, *temp_bytes, cost, total, 0ull, (size_t)ra.nrows,
                                rocprim::plus<unsigned long long>(), st);
-    RowView v{ra.row_tgt, ra.row_off, ra.src, ra.rng, ra.act_cnt, ra.act_off};
+    RowView v{ra.row_tgt, ra.row_off, ra.src, ra.rng, ra.act_cnt, ra.act_off, ra.stage_ops};
     hipLaunchKernelGGL(k_row_cost, dim3(grid_of(ra.nrows)), dim3(256), 0, st, v, ra.nrows, cost);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess)
@@ -379,7 +421,7 @@ hipError_t items_count(const RowArgs &ra, long long split, int *count, int *off,
     if (temp == nullptr)
         return rocprim::exclusive_scan(nullptr, *temp_bytes, count, off, 0, (size_t)ra.nrows + 1,
                                        rocprim::plus<int>(), st);
-    RowView v{ra.row_tgt, ra.row_off, ra.src, ra.rng, ra.act_cnt, ra.act_off};
+    RowView v{ra.row_tgt, ra.row_off, ra.src, ra.rng, ra.act_cnt, ra.act_off, ra.stage_ops};
     hipLaunchKernelGGL(k_items_count, dim3(grid_of(ra.nrows)), dim3(256), 0, st, v, ra.nrows, split, count);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess)
@@ -390,7 +432,7 @@ hipError_t items_count(const RowArgs &ra, long long split, int *count, int *off,
 
 hipError_t items_emit(const RowArgs &ra, long long split, const int *off, Item *items, unsigned *cost, hipStream_t st)
 {
-    RowView v{ra.row_tgt, ra.row_off, ra.src, ra.rng, ra.act_cnt, ra.act_off};
+    RowView v{ra.row_tgt, ra.row_off, ra.src, ra.rng, ra.act_cnt, ra.act_off, ra.stage_ops};
     hipLaunchKernelGGL(k_items_emit, dim3(grid_of(ra.nrows)), dim3(256), 0, st, v, ra.nrows, split, off, items, cost);
     return hipGetLastError();
 }

@@ -108,12 +108,13 @@ struct pp2p_ctx {
     // items
     DevBuf<unsigned long long> row_cost, total;
     DevBuf<int> item_count, item_off;
-    DevBuf<Item> items_plain, items_q_raw, items_q;
+    DevBuf<Item> items_rows, items_q;     // row order / cost-sorted
     DevBuf<unsigned> cost, cost_sorted;
     DevBuf<int> idx, idx_sorted;
-    int nitems_plain = 0, nitems_q = 0;
-    bool plain_ready = false, queue_ready = false;
-    std::vector<int> h_item_off_plain;
+    int nitems = 0;
+    long long split = 0;
+    bool items_ready = false;
+    std::vector<int> h_item_off;           // per row, into items_rows
     unsigned long long list_pairs = 0;
 
     DevBuf<unsigned> counter;
@@ -195,8 +196,8 @@ void pp2p_destroy(pp2p_ctx *c)
     c->x3.release(); c->posi.release(); c->act.release(); c->act_cnt.release(); c->act_off.release();
     c->ilist.release(); c->task_t.release(); c->task_s.release(); c->key_sorted.release(); c->src.release();
     c->flag.release(); c->rowidx.release(); c->row_tgt.release(); c->row_off.release(); c->row_cost.release();
-    c->total.release(); c->item_count.release(); c->item_off.release(); c->items_plain.release();
-    c->items_q_raw.release(); c->items_q.release(); c->cost.release(); c->cost_sorted.release();
+    c->total.release(); c->item_count.release(); c->item_off.release(); c->items_rows.release();
+    c->items_q.release(); c->cost.release(); c->cost_sorted.release();
     c->idx.release(); c->idx_sorted.release(); c->counter.release(); c->temp.release();
     c->pair_count.release();
     if (c->ev_k0) (void)hipEventDestroy(c->ev_k0);
@@ -227,7 +228,7 @@ int pp2p_set_tuning(pp2p_ctx *c, const pp2p_tuning *t)
             return pp2p_set_params(c, &p);
         }
     }
-    c->queue_ready = false;
+    c->items_ready = false;
     return PP2P_OK;
 }
 
@@ -303,7 +304,7 @@ static int finish_particles(pp2p_ctx *c, long long npart, int nleaf)
     c->have_particles = true;
     c->has_act = false;
     c->ntask = 0;
-    c->csr_ready = c->plain_ready = c->queue_ready = false;
+    c->csr_ready = c->items_ready = false;
     HIPCHK(hipStreamSynchronize(c->stream));
     return PP2P_OK;
 }
@@ -354,7 +355,7 @@ int pp2p_set_active(pp2p_ctx *c, const unsigned char *a, int level)
         return st;
     if (!c->have_particles)
         return fail(PP2P_ERR_STATE, "set_particles before set_active");
-    c->plain_ready = c->queue_ready = false;
+    c->items_ready = false;
     if (!a) {
         c->has_act = false;
         return PP2P_OK;
@@ -413,7 +414,7 @@ static int append_tasks(pp2p_ctx *c, long long n, const int *tt, const int *ts)
         HIPCHK(hipStreamSynchronize(c->stream));   // the caller may reuse its buffers on return
     }
     c->ntask += n;
-    c->csr_ready = c->plain_ready = c->queue_ready = false;
+    c->csr_ready = c->items_ready = false;
     return PP2P_OK;
 }
 
@@ -484,6 +485,8 @@ static RowArgs row_args(pp2p_ctx *c)
     ra.act_cnt = c->has_act ? c->act_cnt.p : nullptr;
     ra.act_off = c->has_act ? c->act_off.p : nullptr;
     ra.nrows = c->nrows;
+    // chunk_model > 0: at most that many i-particles per item (default 256)
+    ra.stage_ops = c->tuning.chunk_model > 0 ? (float)c->tuning.chunk_model : -1.0f;
     return ra;
 }
 
@@ -510,7 +513,11 @@ static int queue_groups(pp2p_ctx *c)
     return per_cu * c->num_cu;
 }
 
-static int build_items(pp2p_ctx *c, bool queue)
+// Work items of the current list: every CSR row cut into i-chunks and source
+// ranges of about `split` pair-interactions.  Two copies: row order (GRID and
+// PER_TARGET forms, h_item_off per row) and sorted by cost, largest first
+// (QUEUE form).
+static int build_items(pp2p_ctx *c)
 {
     const RowArgs ra = row_args(c);
     const int nr = c->nrows;
@@ -518,13 +525,11 @@ static int build_items(pp2p_ctx *c, bool queue)
     HIPCHK(c->total.reserve(1));
     HIPCHK(c->item_count.reserve(nr + 1));
     HIPCHK(c->item_off.reserve(nr + 1));
-    if (nr == 0) {
-        (queue ? c->nitems_q : c->nitems_plain) = 0;
-        c->list_pairs = 0;
-        if (!queue)
-            c->h_item_off_plain.assign(1, 0);
+    c->nitems = 0;
+    c->h_item_off.assign(1, 0);
+    c->list_pairs = 0;
+    if (nr == 0)
         return PP2P_OK;
-    }
     size_t tb1 = 0, tb2 = 0;
     HIPCHK(rows_cost(ra, c->row_cost.p, c->total.p, nullptr, &tb1, c->stream));
     HIPCHK(items_count(ra, 0, c->item_count.p, c->item_off.p, nullptr, &tb2, c->stream));
@@ -538,67 +543,54 @@ static int build_items(pp2p_ctx *c, bool queue)
     HIPCHK(hipStreamSynchronize(c->stream));
     c->list_pairs = total;
 
-    long long split = 0;
-    if (queue) {
-        if (c->tuning.split_pairs > 0) {
-            split = c->tuning.split_pairs;
-        } else {
-            // about 8 items per persistent wave, within [2^14, 2^22] pairs
-            const long long waves = (long long)queue_groups(c) * kWavesPerGroup;
-            split = (long long)(total / (unsigned long long)std::max(1ll, 8 * waves));
-            split = std::max(split, 1ll << 14);
-            split = std::min(split, 1ll << 22);
-        }
+    long long split;
+    if (c->tuning.split_pairs > 0) {
+        split = c->tuning.split_pairs;
+    } else {
+        // about 8 items per persistent wave, within [2^14, 2^22] pairs
+        const long long waves = (long long)queue_groups(c) * kWavesPerGroup;
+        split = (long long)(total / (unsigned long long)std::max(1ll, 8 * waves));
+        split = std::max(split, 1ll << 14);
+        split = std::min(split, 1ll << 22);
     }
     HIPCHK(launch_fill_i32(c->item_count.p + nr, 1, 0, c->stream));
     HIPCHK(items_count(ra, split, c->item_count.p, c->item_off.p, c->temp.p, &tb2, c->stream));
     int nitems = 0;
     HIPCHK(hipMemcpyAsync(&nitems, c->item_off.p + nr, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    if (!queue) {
-        HIPCHK(c->items_plain.reserve(std::max(nitems, 1)));
-        HIPCHK(items_emit(ra, 0, c->item_off.p, c->items_plain.p, nullptr, c->stream));
-        c->h_item_off_plain.resize(nr + 1);
-        HIPCHK(hipMemcpyAsync(c->h_item_off_plain.data(), c->item_off.p, sizeof(int) * (nr + 1),
-                              hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
-        c->nitems_plain = nitems;
-        return PP2P_OK;
-    }
-    HIPCHK(c->items_q_raw.reserve(std::max(nitems, 1)));
+    HIPCHK(c->items_rows.reserve(std::max(nitems, 1)));
     HIPCHK(c->items_q.reserve(std::max(nitems, 1)));
     HIPCHK(c->cost.reserve(std::max(nitems, 1)));
     HIPCHK(c->cost_sorted.reserve(std::max(nitems, 1)));
     HIPCHK(c->idx.reserve(std::max(nitems, 1)));
     HIPCHK(c->idx_sorted.reserve(std::max(nitems, 1)));
-    HIPCHK(items_emit(ra, split, c->item_off.p, c->items_q_raw.p, c->cost.p, c->stream));
+    HIPCHK(items_emit(ra, split, c->item_off.p, c->items_rows.p, c->cost.p, c->stream));
     size_t tb3 = 0;
-    HIPCHK(items_sort(c->cost.p, c->cost_sorted.p, c->idx.p, c->idx_sorted.p, c->items_q_raw.p, c->items_q.p, nitems,
+    HIPCHK(items_sort(c->cost.p, c->cost_sorted.p, c->idx.p, c->idx_sorted.p, c->items_rows.p, c->items_q.p, nitems,
                       nullptr, &tb3, c->stream));
     if ((st = ensure_temp(c, tb3)))
         return st;
     tb3 = c->temp.cap;
-    HIPCHK(items_sort(c->cost.p, c->cost_sorted.p, c->idx.p, c->idx_sorted.p, c->items_q_raw.p, c->items_q.p, nitems,
+    HIPCHK(items_sort(c->cost.p, c->cost_sorted.p, c->idx.p, c->idx_sorted.p, c->items_rows.p, c->items_q.p, nitems,
                       c->temp.p, &tb3, c->stream));
+    c->h_item_off.resize(nr + 1);
+    HIPCHK(hipMemcpyAsync(c->h_item_off.data(), c->item_off.p, sizeof(int) * (nr + 1), hipMemcpyDeviceToHost,
+                          c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    c->nitems_q = nitems;
+    c->nitems = nitems;
+    c->split = split;
     return PP2P_OK;
 }
 
-static int prepare_form(pp2p_ctx *c, bool queue)
+static int prepare_form(pp2p_ctx *c, bool)
 {
     int st;
     if (!c->csr_ready && (st = build_csr(c)))
         return st;
-    if (queue && !c->queue_ready) {
-        if ((st = build_items(c, true)))
+    if (!c->items_ready) {
+        if ((st = build_items(c)))
             return st;
-        c->queue_ready = true;
-    }
-    if (!queue && !c->plain_ready) {
-        if ((st = build_items(c, false)))
-            return st;
-        c->plain_ready = true;
+        c->items_ready = true;
     }
     return PP2P_OK;
 }
@@ -611,7 +603,7 @@ int pp2p_prepare(pp2p_ctx *c)
     if (!c->have_particles)
         return fail(PP2P_ERR_STATE, "set_particles before prepare");
     HIPCHK(hipEventRecord(c->ev_p0, c->stream));
-    c->csr_ready = c->queue_ready = false;
+    c->csr_ready = c->items_ready = false;
     if ((st = prepare_form(c, true)))
         return st;
     HIPCHK(hipEventRecord(c->ev_p1, c->stream));
@@ -624,8 +616,8 @@ static ForceArgs force_args(pp2p_ctx *c, bool queue)
 {
     ForceArgs a;
     std::memset(&a, 0, sizeof(a));
-    a.items = queue ? c->items_q.p : c->items_plain.p;
-    a.nitems = queue ? c->nitems_q : c->nitems_plain;
+    a.items = queue ? c->items_q.p : c->items_rows.p;
+    a.nitems = c->nitems;
     a.item_base = 0;
     a.counter = c->counter.p;
     a.pos = c->pos.p;
@@ -659,7 +651,7 @@ static int enqueue_compute(pp2p_ctx *c, int exec)
         HIPCHK(launch_force(a, c->law.lc, v, (a.nitems + kWavesPerGroup - 1) / kWavesPerGroup, c->stream));
     } else {
         for (int r = 0; r < c->nrows; ++r) {
-            const int b = c->h_item_off_plain[r], e = c->h_item_off_plain[r + 1];
+            const int b = c->h_item_off[r], e = c->h_item_off[r + 1];
             if (e <= b)
                 continue;
             a.item_base = b;
@@ -805,7 +797,7 @@ int pp2p_list_stats(pp2p_ctx *c, long long *ntask, long long *nrows, long long *
         return st;
     if (ntask) *ntask = c->ntask;
     if (nrows) *nrows = c->nrows;
-    if (nitems) *nitems = c->nitems_q;
+    if (nitems) *nitems = c->nitems;
     if (npairs) *npairs = (long long)c->list_pairs;
     return PP2P_OK;
 }

@@ -14,7 +14,8 @@ constexpr int kLogOctaves = 32;      // LOGTAB: cyclic table spans 32 binary oct
 constexpr int kLogSegsMax = 32;      // LOGTAB: segments per octave (16 or 32)
 constexpr int kLogTabMax = kLogOctaves * kLogSegsMax;
 constexpr int kPolyDeg = 12;         // POLY: degree of h(r)
-constexpr int kItemMaxI = 64;        // i-particles per work item (one per lane)
+constexpr int kIplMax = 4;           // i-particles per lane, at most
+constexpr int kItemMaxI = kWave * kIplMax;   // i-particles per work item
 
 // Units inside the kernels: lengths are divided by the cutoff radius rc
 // (u = ucut * r / rc), so the cutoff sits at s = r^2 = 1 exactly.
@@ -27,11 +28,12 @@ enum { LAW_LOGTAB = 0, LAW_POLY = 1 };
 struct alignas(16) Item {
     int tgt;      // target leaf id
     int i_beg;    // first i-particle (index into pos) or into ilist
-    int ni;       // number of i-particles, 1..64
+    int ni;       // number of i-particles, 1..256
     int row;      // CSR row
     int s_beg;    // source range [s_beg, s_end) into src[]
     int s_end;
-    int pad0, pad1;
+    int pad0;     // log2 of the number of phases G
+    int pad1;     // i-particles per lane (IPL), 1..4
 };
 
 // Force-law constants, passed by value.

@@ -26,6 +26,15 @@ __device__ __forceinline__ void wave_lds_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
+// lane k's double, as a wave-uniform value
+__device__ __forceinline__ double rl_double(double v, int k)
+{
+    const unsigned long long b = __double_as_longlong(v);
+    const unsigned lo = __builtin_amdgcn_readlane((unsigned)b, k);
+    const unsigned hi = __builtin_amdgcn_readlane((unsigned)(b >> 32), k);
+    return __longlong_as_double(((unsigned long long)hi << 32) | lo);
+}
+
 __device__ __forceinline__ float wave_min(float v)
 {
 #pragma unroll
@@ -43,12 +52,13 @@ __device__ __forceinline__ float wave_max(float v)
 }
 
 // s = r_eff^2 / rc^2 with the softening rule, clamped to s <= 1 (the cutoff).
+// Plummer folds eps^2 into the first product of r^2 (one FMA, no extra add).
 template <int SOFT>
-__device__ __forceinline__ float soft_s(float r2, const LawConst &lc)
+__device__ __forceinline__ float soft_s(float dx, float dy, float dz, const LawConst &lc)
 {
     if (SOFT == SOFT_REF)
-        return __builtin_amdgcn_fmed3f(r2, lc.smin, 1.0f);     // max(r, eps), clamp
-    return fminf(r2 + lc.smin, 1.0f);
+        return __builtin_amdgcn_fmed3f(fmaf(dz, dz, fmaf(dy, dy, dx * dx)), lc.smin, 1.0f);   // max(r, eps), clamp
+    return fminf(fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, lc.smin))), 1.0f);
 }
 
 // One pair, LOGTAB law.  F(s) = g(u(s)) / s^{3/2} is a cubic in (s - s0) on
@@ -65,8 +75,7 @@ __device__ __forceinline__ void pair_logtab(float xi, float yi, float zi, float4
     const float dx = pj.x - xi;
     const float dy = pj.y - yi;
     const float dz = pj.z - zi;
-    const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-    const float s = soft_s<SOFT>(r2, lc);
+    const float s = soft_s<SOFT>(dx, dy, dz, lc);
     const unsigned bits = __float_as_uint(s);
     const unsigned seg = __builtin_amdgcn_ubfe(bits, lc.seg_shift, lc.seg_bits);
     const float s0 = __uint_as_float(bits & (0xffffffffu << lc.seg_shift));
@@ -88,8 +97,7 @@ __device__ __forceinline__ void pair_poly(float xi, float yi, float zi, float4 p
     const float dx = pj.x - xi;
     const float dy = pj.y - yi;
     const float dz = pj.z - zi;
-    const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-    const float s = soft_s<SOFT>(r2, lc);
+    const float s = soft_s<SOFT>(dx, dy, dz, lc);
     const float rinv = __builtin_amdgcn_rsqf(s);
     const float r = s * rinv;
     const float e = __builtin_amdgcn_exp2f(s * lc.k2);
@@ -108,10 +116,216 @@ __device__ __forceinline__ void pair_poly(float xi, float yi, float zi, float4 p
 // the number of phases and contributes exactly zero.
 __device__ __forceinline__ float4 null_particle() { return make_float4(1e18f, 1e18f, 1e18f, 0.f); }
 
-// FOLD64: leaf frames, FP64 fold of every batch's FP32 sums; otherwise one
-// FP32 accumulator over absolute coordinates (reference-style form).
+#ifndef PP2P_WAVES_PER_EU
+#define PP2P_WAVES_PER_EU 4
+#endif
+
+// One work item with IPL i-particles per lane.  Lane l belongs to phase
+// p = l / L (L = 64 / G lanes per phase, G = 2^glog phases) and holds the
+// i-slots q, q + L, ..., q + (IPL-1) L of the item, q = l % L; slots past
+// n_i hold a far-away dummy.  Source particles are staged into the tile in
+// batches of 64; phase p takes tile entries p, p + G, p + 2G, ... (lanes of
+// one phase read one address, neighbouring phases neighbouring addresses), so
+// every j read feeds IPL independent pairs.  The G partial sums of an
+// i-particle meet by a butterfly over lanes.
+template <int IPL, int LAW, int SOFT, bool FOLD64, bool INDIRECT, bool COUNT>
+__device__ __forceinline__ void process_item(const ForceArgs &a, const LawConst &lc, const Item &it,
+                                             const float4 *__restrict__ s_tab, float4 *tile, int lane,
+                                             unsigned long long &n_in_total)
+{
+    const int ni = it.ni;
+    const int glog = it.pad0;
+    const int G = 1 << glog;
+    const int L = kWave >> glog;
+    const int phase = lane >> (6 - glog);
+    const int q = lane & (L - 1);
+    const unsigned long long below = (1ull << lane) - 1ull;
+
+    float xi[IPL], yi[IPL], zi[IPL];
+    int ip[IPL];
+    bool valid[IPL];
+    float bxl = 3.0e38f, bxh = -3.0e38f, byl = 3.0e38f, byh = -3.0e38f, bzl = 3.0e38f, bzh = -3.0e38f;
+#pragma unroll
+    for (int k = 0; k < IPL; ++k) {
+        const int il = q + k * L;
+        valid[k] = il < ni;
+        ip[k] = 0;
+        xi[k] = yi[k] = zi[k] = 1e18f;
+        if (valid[k]) {
+            ip[k] = INDIRECT ? a.ilist[it.i_beg + il] : it.i_beg + il;
+            const float4 p = a.pos[ip[k]];
+            xi[k] = p.x; yi[k] = p.y; zi[k] = p.z;
+            bxl = fminf(bxl, p.x); bxh = fmaxf(bxh, p.x);
+            byl = fminf(byl, p.y); byh = fmaxf(byh, p.y);
+            bzl = fminf(bzl, p.z); bzh = fmaxf(bzh, p.z);
+        }
+    }
+    // bounding box of the item's i-particles (lane 0 always holds a valid slot)
+    bxl = wave_min(bxl); bxh = wave_max(bxh);
+    byl = wave_min(byl); byh = wave_max(byh);
+    bzl = wave_min(bzl); bzh = wave_max(bzh);
+    const float cx = 0.5f * (bxl + bxh), hx = 0.5f * (bxh - bxl);
+    const float cy = 0.5f * (byl + byh), hy = 0.5f * (byh - byl);
+    const float cz = 0.5f * (bzl + bzh), hz = 0.5f * (bzh - bzl);
+    const float cull_lim = a.cull ? 1.0009765625f : 3.0e38f;   // s > 1 + 2^-10: no pair can reach
+
+    const double4 rt = a.ref[it.tgt];
+    double sx[IPL], sy[IPL], sz[IPL];
+    float fx[IPL], fy[IPL], fz[IPL];
+#pragma unroll
+    for (int k = 0; k < IPL; ++k) {
+        sx[k] = sy[k] = sz[k] = 0.0;
+        fx[k] = fy[k] = fz[k] = 0.f;
+    }
+    unsigned n_in = 0;
+    int cnt = 0;                                   // j-particles waiting in the tile
+
+    auto run_batch = [&](int n) {                  // n: multiple of G
+        const int ch = n >> glog;
+        const float4 *tp = tile + phase;
+#pragma unroll 2
+        for (int u = 0; u < ch; ++u) {
+            const float4 pj = tp[u << glog];
+#pragma unroll
+            for (int k = 0; k < IPL; ++k) {
+                if (COUNT) {
+                    const float dx = pj.x - xi[k], dy = pj.y - yi[k], dz = pj.z - zi[k];
+                    n_in += valid[k] && soft_s<SOFT>(dx, dy, dz, lc) < 1.0f;
+                } else if (LAW == LAW_LOGTAB) {
+                    pair_logtab<SOFT>(xi[k], yi[k], zi[k], pj, s_tab, lc, fx[k], fy[k], fz[k]);
+                } else {
+                    pair_poly<SOFT>(xi[k], yi[k], zi[k], pj, lc, fx[k], fy[k], fz[k]);
+                }
+            }
+        }
+        if (FOLD64) {
+#pragma unroll
+            for (int k = 0; k < IPL; ++k) {
+                sx[k] += (double)fx[k]; sy[k] += (double)fy[k]; sz[k] += (double)fz[k];
+                fx[k] = 0.f; fy[k] = 0.f; fz[k] = 0.f;
+            }
+        }
+    };
+
+    // Source metadata 64 sources at a time (one per lane, read back with
+    // readlane); particle chunks of 64 prefetched one chunk ahead so the
+    // global-load latency overlaps the pair loop of the current tile.
+    for (int sb = it.s_beg; sb < it.s_end; sb += kWave) {
+        const int nsrc = min(kWave, it.s_end - sb);
+        int m_ip = 0, m_np = 0;
+        double m_ox = 0.0, m_oy = 0.0, m_oz = 0.0;
+        if (lane < nsrc) {
+            const int sl = a.src[sb + lane];
+            const int2 jr = a.rng[sl];
+            m_ip = jr.x;
+            m_np = jr.y;
+            if (FOLD64) {
+                const double4 rsrc = a.ref[sl];
+                m_ox = rsrc.x - rt.x; m_oy = rsrc.y - rt.y; m_oz = rsrc.z - rt.z;
+                if (a.box > 0.0) {
+                    m_ox -= a.box * rint(m_ox / a.box);
+                    m_oy -= a.box * rint(m_oy / a.box);
+                    m_oz -= a.box * rint(m_oz / a.box);
+                }
+            }
+        }
+        int k = 0, c0 = 0;
+        int ip_k = __builtin_amdgcn_readlane(m_ip, 0);
+        int np_k = __builtin_amdgcn_readlane(m_np, 0);
+        float4 qv = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (lane < np_k)
+            qv = a.pos[ip_k + lane];
+        while (k < nsrc) {
+            // next chunk position and its prefetch
+            int k2 = k, c2 = c0 + kWave;
+            if (c2 >= np_k) {
+                k2 = k + 1;
+                c2 = 0;
+            }
+            int ip2 = 0, np2 = 0;
+            float4 q2 = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (k2 < nsrc) {
+                ip2 = __builtin_amdgcn_readlane(m_ip, k2);
+                np2 = __builtin_amdgcn_readlane(m_np, k2);
+                if (lane < np2 - c2)
+                    q2 = a.pos[ip2 + c2 + lane];
+            }
+            // stage the current chunk: target frame, cull, compact
+            const bool has = lane < np_k - c0;
+            float4 w = qv;
+            if (FOLD64) {
+                const double ox = rl_double(m_ox, k), oy = rl_double(m_oy, k), oz = rl_double(m_oz, k);
+                w.x = (float)((double)qv.x + ox);
+                w.y = (float)((double)qv.y + oy);
+                w.z = (float)((double)qv.z + oz);
+            }
+            const float ex = fmaxf(fabsf(w.x - cx) - hx, 0.f);
+            const float ey = fmaxf(fabsf(w.y - cy) - hy, 0.f);
+            const float ez = fmaxf(fabsf(w.z - cz) - hz, 0.f);
+            const bool keep = has && fmaf(ez, ez, fmaf(ey, ey, ex * ex)) <= cull_lim;
+            const unsigned long long m = __ballot(keep);
+            if (keep)
+                tile[cnt + __popcll(m & below)] = make_float4(w.x, w.y, w.z, 0.f);
+            cnt += __popcll(m);
+            wave_lds_sync();
+            if (cnt >= kWave) {
+                run_batch(kWave);
+                wave_lds_sync();
+                const int rem = cnt - kWave;           // < 64: no overlap
+                float4 v;
+                if (lane < rem)
+                    v = tile[kWave + lane];
+                wave_lds_sync();
+                if (lane < rem)
+                    tile[lane] = v;
+                cnt = rem;
+                wave_lds_sync();
+            }
+            qv = q2;
+            if (k2 != k) {
+                ip_k = ip2;
+                np_k = np2;
+            }
+            k = k2;
+            c0 = c2;
+        }
+    }
+    if (cnt > 0) {
+        const int n = ((cnt + G - 1) >> glog) << glog;
+        if (lane < n - cnt)
+            tile[cnt + lane] = null_particle();
+        wave_lds_sync();
+        run_batch(n);
+        wave_lds_sync();
+    }
+    if (COUNT) {
+        n_in_total += n_in;
+        return;
+    }
+#pragma unroll
+    for (int k = 0; k < IPL; ++k) {
+        if (!FOLD64) {
+            sx[k] = (double)fx[k]; sy[k] = (double)fy[k]; sz[k] = (double)fz[k];
+        }
+        // butterfly over the phases: lanes q, q + L, q + 2L, ... hold slot k of one i
+        for (int s = L; s < kWave; s <<= 1) {
+            sx[k] += __shfl_xor(sx[k], s, kWave);
+            sy[k] += __shfl_xor(sy[k], s, kWave);
+            sz[k] += __shfl_xor(sz[k], s, kWave);
+        }
+        if (phase == 0 && valid[k]) {
+            unsafeAtomicAdd(a.acc + ip[k], sx[k] * a.out_scale);
+            unsafeAtomicAdd(a.acc + a.npart + ip[k], sy[k] * a.out_scale);
+            unsafeAtomicAdd(a.acc + 2 * a.npart + ip[k], sz[k] * a.out_scale);
+        }
+    }
+}
+
+// Persistent (QUEUE) or one-wave-per-item kernel.  The item's IPL selects
+// one of four specialisations; the branch is uniform per wave.
 template <int LAW, int SOFT, bool FOLD64, bool INDIRECT, bool QUEUE, bool COUNT = false>
 __global__ __launch_bounds__(kWave * kWavesPerGroup)
+__attribute__((amdgpu_waves_per_eu(PP2P_WAVES_PER_EU)))
 void p2p_force(ForceArgs a, LawConst lc)
 {
     extern __shared__ float4 s_tab[];                       // LOGTAB table
@@ -127,8 +341,8 @@ void p2p_force(ForceArgs a, LawConst lc)
     }
 
     float4 *tile = s_tile[wv];
-    const unsigned long long below = (1ull << lane) - 1ull;
     int item_id = QUEUE ? 0 : a.item_base + blockIdx.x * kWavesPerGroup + wv;
+    unsigned long long n_in = 0;
 
     for (;;) {
         if (QUEUE) {
@@ -140,148 +354,20 @@ void p2p_force(ForceArgs a, LawConst lc)
         if (item_id >= a.nitems)
             break;
         const Item it = a.items[item_id];
-        const int ni = it.ni;
-        const int groups = kWave / ni;                  // phases, >= 1
-        const int batch = groups * (kWave / groups);    // j per batch, 60..64
-        // lane / ni for lane < 64 and ni <= 64 without a divide
-        const int phase = (lane * ((65536 + ni - 1) / ni)) >> 16;
-        const int il = lane - phase * ni;               // < ni for every lane
-        const bool live = phase < groups;
-
-        const int ip = INDIRECT ? a.ilist[it.i_beg + il] : it.i_beg + il;
-        const float4 pi = a.pos[ip];
-        const float xi = pi.x, yi = pi.y, zi = pi.z;
-
-        // bounding box of the item's i-particles (all lanes hold some i)
-        const float bxl = wave_min(xi), bxh = wave_max(xi);
-        const float byl = wave_min(yi), byh = wave_max(yi);
-        const float bzl = wave_min(zi), bzh = wave_max(zi);
-        const float cx = 0.5f * (bxl + bxh), hx = 0.5f * (bxh - bxl);
-        const float cy = 0.5f * (byl + byh), hy = 0.5f * (byh - byl);
-        const float cz = 0.5f * (bzl + bzh), hz = 0.5f * (bzh - bzl);
-        const float cull_lim = a.cull ? 1.0009765625f : 3.0e38f;   // s > 1 + 2^-10: no pair can reach
-
-        const double4 rt = a.ref[it.tgt];
-        double sx = 0.0, sy = 0.0, sz = 0.0;
-        float fx = 0.f, fy = 0.f, fz = 0.f;
-        int cnt = 0;                                   // j-particles waiting in the tile
-
-        unsigned n_in = 0;                             // COUNT: pairs inside the cutoff
-        auto run_batch = [&](int n) {                  // n: multiple of groups
-            const int ch = n / groups;
-            if (live) {
-                const float4 *tp = tile + phase * ch;
-#pragma unroll 4
-                for (int u = 0; u < ch; ++u) {
-                    const float4 pj = tp[u];
-                    if (COUNT) {
-                        const float dx = pj.x - xi, dy = pj.y - yi, dz = pj.z - zi;
-                        n_in += soft_s<SOFT>(fmaf(dz, dz, fmaf(dy, dy, dx * dx)), lc) < 1.0f;
-                    } else if (LAW == LAW_LOGTAB)
-                        pair_logtab<SOFT>(xi, yi, zi, pj, s_tab, lc, fx, fy, fz);
-                    else
-                        pair_poly<SOFT>(xi, yi, zi, pj, lc, fx, fy, fz);
-                }
-            }
-            if (FOLD64) {
-                sx += (double)fx; sy += (double)fy; sz += (double)fz;
-                fx = 0.f; fy = 0.f; fz = 0.f;
-            }
-        };
-
-        for (int s = it.s_beg; s < it.s_end; ++s) {
-            const int sl = a.src[s];
-            const int2 jr = a.rng[sl];
-            double ox = 0.0, oy = 0.0, oz = 0.0;
-            if (FOLD64) {
-                const double4 rsrc = a.ref[sl];
-                ox = rsrc.x - rt.x; oy = rsrc.y - rt.y; oz = rsrc.z - rt.z;
-                if (a.box > 0.0) {
-                    ox -= a.box * rint(ox / a.box);
-                    oy -= a.box * rint(oy / a.box);
-                    oz -= a.box * rint(oz / a.box);
-                }
-            }
-            for (int c0 = 0; c0 < jr.y; c0 += kWave) {
-                const bool has = lane < jr.y - c0;
-                float4 w = make_float4(0.f, 0.f, 0.f, 0.f);
-                if (has) {
-                    const float4 q = a.pos[jr.x + c0 + lane];
-                    if (FOLD64) {
-                        w.x = (float)((double)q.x + ox);
-                        w.y = (float)((double)q.y + oy);
-                        w.z = (float)((double)q.z + oz);
-                    } else {
-                        w.x = q.x; w.y = q.y; w.z = q.z;
-                    }
-                }
-                const float ex = fmaxf(fabsf(w.x - cx) - hx, 0.f);
-                const float ey = fmaxf(fabsf(w.y - cy) - hy, 0.f);
-                const float ez = fmaxf(fabsf(w.z - cz) - hz, 0.f);
-                const bool keep = has && fmaf(ez, ez, fmaf(ey, ey, ex * ex)) <= cull_lim;
-                const unsigned long long m = __ballot(keep);
-                if (keep)
-                    tile[cnt + __popcll(m & below)] = w;
-                cnt += __popcll(m);
-                wave_lds_sync();
-                while (cnt >= batch) {
-                    run_batch(batch);
-                    wave_lds_sync();
-                    const int rem = cnt - batch;       // < batch: no overlap
-                    float4 v;
-                    if (lane < rem)
-                        v = tile[batch + lane];
-                    wave_lds_sync();
-                    if (lane < rem)
-                        tile[lane] = v;
-                    cnt = rem;
-                    wave_lds_sync();
-                }
-            }
-        }
-        if (cnt > 0) {
-            const int n = groups * ((cnt + groups - 1) / groups);
-            if (lane < n - cnt)
-                tile[cnt + lane] = null_particle();
-            wave_lds_sync();
-            run_batch(n);
-            wave_lds_sync();
-        }
-        if (!FOLD64) {
-            sx = (double)fx; sy = (double)fy; sz = (double)fz;
-        }
-        if (groups > 1) {                               // sum the phases (tile is free now)
-            double *red = reinterpret_cast<double *>(tile);
-            red[lane] = sx;
-            red[kWave + lane] = sy;
-            red[2 * kWave + lane] = sz;
-            wave_lds_sync();
-            if (phase == 0) {
-                for (int g = 1; g < groups; ++g) {
-                    sx += red[lane + g * ni];
-                    sy += red[kWave + lane + g * ni];
-                    sz += red[2 * kWave + lane + g * ni];
-                }
-            }
-            wave_lds_sync();
-        }
-        if (COUNT) {
-            unsigned long long c = n_in;
-            for (int o = 32; o > 0; o >>= 1)
-                c += __shfl_xor(c, o, kWave);
-            if (lane == 0)
-                atomicAdd(a.count, c);
-            if (!QUEUE)
-                break;
-            continue;
-        }
-        if (phase == 0) {
-            unsafeAtomicAdd(a.acc + ip, sx * a.out_scale);
-            unsafeAtomicAdd(a.acc + a.npart + ip, sy * a.out_scale);
-            unsafeAtomicAdd(a.acc + 2 * a.npart + ip, sz * a.out_scale);
+        switch (it.pad1) {
+        case 1: process_item<1, LAW, SOFT, FOLD64, INDIRECT, COUNT>(a, lc, it, s_tab, tile, lane, n_in); break;
+        case 2: process_item<2, LAW, SOFT, FOLD64, INDIRECT, COUNT>(a, lc, it, s_tab, tile, lane, n_in); break;
+        case 3: process_item<3, LAW, SOFT, FOLD64, INDIRECT, COUNT>(a, lc, it, s_tab, tile, lane, n_in); break;
+        default: process_item<4, LAW, SOFT, FOLD64, INDIRECT, COUNT>(a, lc, it, s_tab, tile, lane, n_in); break;
         }
         if (!QUEUE)
             break;
+    }
+    if (COUNT) {
+        for (int o = 32; o > 0; o >>= 1)
+            n_in += __shfl_xor(n_in, o, kWave);
+        if (lane == 0)
+            atomicAdd(a.count, n_in);
     }
 }
 

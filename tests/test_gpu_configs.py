@@ -81,8 +81,9 @@ def test_precision_forms(gpu_ctx):
 def test_exec_forms_agree_on_tree_lists(gpu_ctx):
     w = W.config_plain(20)
     outs = [_ctx_run(gpu_ctx, w, exec_form=e) for e in (pp.EXEC_QUEUE, pp.EXEC_GRID, pp.EXEC_PER_TARGET)]
-    # GRID and PER_TARGET run the same unsplit items: identical sums
-    np.testing.assert_array_equal(outs[1], outs[2])
+    # GRID and PER_TARGET run the same items; split targets meet in FP64
+    # atomics whose order varies: equal to FP64 rounding
+    np.testing.assert_allclose(outs[1], outs[2], rtol=1e-11, atol=1e-13 * np.abs(outs[1]).max())
     # the queue splits long source lists: FP32 partial sums regroup
     med, p99, mx = pyoracle.rel_err(outs[0], outs[1])
     print(f"queue vs grid: median {med:.2e} p99 {p99:.2e} max {mx:.2e}")
