@@ -46,7 +46,10 @@ def log(*a):
 def make_workload(name: str, n1: int | None, threads: int):
     from photons_p2p import workloads as W
     if name == "clustered":
-        return W.config_clustered(n1 or 256, box=400.0 * (n1 or 256) / 256, nthreads=threads)
+        n = n1 or 256
+        # 256^3 in 400 Mpc/h; 512^3 in 1 Gpc/h (the large single-GPU share)
+        box = 1000.0 if n == 512 else 400.0 * n / 256
+        return W.config_clustered(n, box=box, nthreads=threads)
     if name == "plain":
         return W.config_plain(n1 or 64, nthreads=threads)
     if name == "precision":
@@ -64,13 +67,16 @@ def sample_rows(w, nrows: int, seed: int):
     return np.isin(w.task_t, pick)
 
 
-def accuracy(w, acc, mask, threads):
+def accuracy(w, acc, mask, threads, pos=None, act=None, level=1):
     import pyoracle
     prm = pyoracle.params(w.eps, w.rs, mass=w.mass, softening=w.softening, box=w.box)
     csr = pyoracle.build_csr(w.task_t[mask], w.task_s[mask], w.nleaf)
-    ref = pyoracle.p2p_f64(prm, w.pos, w.leaf_ip, w.leaf_np, csr, nthreads=threads)
+    ref = pyoracle.p2p_f64(prm, w.pos if pos is None else pos, w.leaf_ip, w.leaf_np, csr, act=act, level=level,
+                           nthreads=threads)
     tgt = csr[0]
     idx = np.concatenate([np.arange(w.leaf_ip[t], w.leaf_ip[t] + w.leaf_np[t]) for t in tgt])
+    if act is not None:
+        idx = idx[act[idx] == level]
     return pyoracle.rel_err(acc[idx], ref[idx]), idx.size
 
 
@@ -114,6 +120,13 @@ def main():
     ap.add_argument("--no-cull", action="store_true")
     ap.add_argument("--graph", type=int, default=0,
                     help="capture this many steps in one hipGraph and time its replays (0: eager steps)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="measure the tuning knobs on this workload first (else use photons_p2p/tuned.json)")
+    ap.add_argument("--tuned-out", default=None, help="write the autotuned entry into this JSON table")
+    ap.add_argument("--active", type=float, default=0.0,
+                    help="active fraction (0: every particle); targets with an active particle only")
+    ap.add_argument("--precision-forms", action="store_true",
+                    help="precision workload: also evaluate FP32 absolute coordinates and report both errors")
     ap.add_argument("--compare-plain", action="store_true",
                     help="also time the one-launch-per-target form the same way and report the ratio")
     ap.add_argument("--chunk-model", type=int, default=0, help="i-chunking: 0 default, -1 chunks of 64, >0 cost")
@@ -130,6 +143,17 @@ def main():
     w = make_workload(args.workload, args.n1, threads)
     log(f"[bench] workload {w.name}: N={w.npart} leaves={w.nleaf} tasks={w.task_t.size} "
         f"built in {time.time() - t0:.1f}s")
+    act = None
+    if args.active > 0:
+        from photons_p2p import workloads as W
+        n_grid = (w.meta["n1"] ** 3) // 2 if w.name.startswith("clustered") else None
+        act, tmask = W.active_subset(w, args.active, n_grid=n_grid)
+        w.task_t, w.task_s = w.task_t[tmask], w.task_s[tmask]
+        log(f"[bench] active fraction {args.active}: {int((act == 1).sum())} active particles, "
+            f"{w.task_t.size} tasks to targets holding one")
+    pos_eval = w.pos
+    if args.workload == "precision":
+        pos_eval = w.pos + w.meta["offset"]      # absolute coordinates ~1e3 box lengths from the origin
     exec_form = {"queue": pp.EXEC_QUEUE, "grid": pp.EXEC_GRID, "per_target": pp.EXEC_PER_TARGET}[args.exec]
     law = pp.LAW_TABLE if args.law == "table" else pp.LAW_POLY
 
@@ -137,10 +161,31 @@ def main():
     ctx.set_law(pp.ForceLaw(eps=w.eps, rs=w.rs, mass=w.mass, softening=w.softening, law=law, box=w.box))
     ctx.set_tuning(waves_per_cu=args.waves_per_cu, split_pairs=args.split, segs_per_octave=args.segs,
                    no_cull=int(args.no_cull), chunk_model=args.chunk_model)
-    ctx.set_particles(w.pos, w.leaf_ip, w.leaf_np)
+    ctx.set_particles(pos_eval, w.leaf_ip, w.leaf_np)
+    if act is not None:
+        ctx.set_active(act, 1)
     ctx.set_tasks(w.task_t, w.task_s)
     prep_ms = ctx.prepare()
     st = ctx.list_stats()
+    from photons_p2p import autotune as AT
+    sig = AT.signature(st["ntask"], w.npart, w.nleaf, st["npairs"])
+    lawp = pp.ForceLaw(eps=w.eps, rs=w.rs, mass=w.mass, softening=w.softening, law=law, box=w.box)
+    tuned = None
+    explicit = args.waves_per_cu or args.split or args.chunk_model or args.segs or args.no_cull
+    if args.autotune:
+        res = AT.tune(ctx, lawp, log=log)
+        tuned = {"source": "autotune", "signature": sig, **res}
+        log(f"[bench] autotuned {sig}: {res['best']} ({res['ms']:.3f} ms/step)")
+        if args.tuned_out:
+            table = AT.load_table(args.tuned_out)
+            table[sig] = {"best": res["best"], "ms": res["ms"], "workload": w.name}
+            AT.save_table(table, args.tuned_out)
+    elif not explicit:
+        cfg = AT.lookup(sig)
+        if cfg is not None:
+            AT.apply(ctx, lawp, cfg)
+            tuned = {"source": "photons_p2p/tuned.json", "signature": sig, "best": cfg}
+            log(f"[bench] tuned.json {sig}: {cfg}")
     n_list = st["npairs"]
     n_in = ctx.count_inside()
     log(f"[bench] prepared in {prep_ms:.1f} ms: rows={st['nrows']} items={st['nitems']} "
@@ -181,13 +226,27 @@ def main():
 
     ctx_tuning = ctx.get_tuning()
     rel = None
+    forms = None
     if not args.no_acc:
         acc = ctx.acc()
         mask = sample_rows(w, args.acc_rows, seed=20261202)
-        (med, p99, mx), nchk = accuracy(w, acc, mask, threads)
+        (med, p99, mx), nchk = accuracy(w, acc, mask, threads, pos=pos_eval, act=act)
         rel = {"median": med, "p99": p99, "max": mx, "particles": int(nchk),
                "sample": f"{args.acc_rows} target leaves, FP64 oracle"}
         log(f"[bench] rel err vs FP64 oracle ({nchk} particles): median {med:.2e} p99 {p99:.2e} max {mx:.2e}")
+        if args.precision_forms:
+            ctx.set_law(pp.ForceLaw(eps=w.eps, rs=w.rs, mass=w.mass, softening=w.softening, law=law,
+                                    frame=pp.FRAME_ABS32, box=w.box))
+            ctx.set_particles(pos_eval, w.leaf_ip, w.leaf_np)
+            if act is not None:
+                ctx.set_active(act, 1)
+            ctx.set_tasks(w.task_t, w.task_s)
+            ctx.compute(exec_form)
+            (m2, p2, x2), _ = accuracy(w, ctx.acc(), mask, threads, pos=pos_eval, act=act)
+            forms = {"fp32_leaf_relative_fp64_offsets": rel,
+                     "fp32_absolute": {"median": m2, "p99": p2, "max": x2},
+                     "oracle": "FP64 direct sum, same lists and force law"}
+            log(f"[bench] FP32 absolute coordinates: median {m2:.2e} p99 {p2:.2e} max {x2:.2e}")
     ctx.close()
 
     cpu = None
@@ -212,6 +271,7 @@ def main():
         "dtype": "fp32",
         "data": "synthetic",
         "tuning": ctx_tuning,
+        "autotune": tuned,
         "config": {"workload": w.name, "npart": int(w.npart), "nleaf": int(w.nleaf), "ntask": int(w.task_t.size),
                    "n_list": int(n_list), "n_in": int(n_in), "exec": args.exec, "law": args.law,
                    "softening": "plummer" if w.softening == 1 else "ref", "eps": w.eps, "rs": w.rs},
@@ -219,6 +279,8 @@ def main():
         "rel_err": rel,
         "prepare_ms": prep_ms,
         "graph_steps": args.graph,
+        "active_fraction": args.active if args.active > 0 else None,
+        "precision_forms": forms,
         "plain_form": plain,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None},

@@ -90,7 +90,9 @@ typedef struct {
     int chunk_model;      /* largest i-chunk of a target per work item: <= 0 the
                              kernel maximum of 256 (64 lanes x 4 per lane), > 0
                              at most this many (balanced chunks) */
-    int reserved[3];
+    int ipl_max;          /* most i-particles per lane an item may use, 1..4
+                             (0 = 4); fewer trades idle lanes for registers */
+    int reserved[2];
 } pp2p_tuning;
 
 typedef struct pp2p_ctx pp2p_ctx;

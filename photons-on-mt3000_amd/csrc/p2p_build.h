@@ -12,7 +12,8 @@ struct RowArgs {
     const int *act_cnt;   // per leaf active count (null: all particles)
     const int *act_off;   // per leaf offset into ilist
     int nrows;
-    float stage_ops;      // i-chunking model: staging cost per source particle (< 0: 64-chunks)
+    float stage_ops;      // > 0: most i-particles per item (default 64 * ipl_max)
+    int ipl_max;          // most i-particles per lane, 1..4
 };
 
 hipError_t launch_leaf_frame(const double *x3, const int2 *rng, int nleaf, double inv_rc, bool frame_abs,

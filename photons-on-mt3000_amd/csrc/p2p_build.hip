@@ -165,7 +165,8 @@ struct RowView {
     const int *row_tgt, *row_off, *src;
     const int2 *rng;
     const int *act_cnt, *act_off;   // null: all particles of the target
-    float stage_ops;                 // chunking model (< 0: 64-particle chunks)
+    float stage_ops;                 // > 0: most i-particles per item
+    int ipl_max;                     // most i-particles per lane, 1..4
 };
 
 __device__ __forceinline__ void row_i(const RowView &v, int tgt, int &ibeg, int &ni)
@@ -199,7 +200,7 @@ __global__ void k_row_cost(RowView v, int nrows, unsigned long long *__restrict_
 // wave time per source particle is proportional to IPL / G; the smallest value
 // wins, ties go to fewer phases.  48 -> G 4 x IPL 3 (all lanes busy),
 // 24 -> 8 x 3, 64 -> 1 x 1, 1 -> 64 x 1.
-__device__ __forceinline__ void item_shape(int c, int &glog, int &ipl)
+__device__ __forceinline__ void item_shape(int c, int ipl_max, int &glog, int &ipl)
 {
     glog = 0;
     ipl = (c + kWave - 1) / kWave;
@@ -207,7 +208,7 @@ __device__ __forceinline__ void item_shape(int c, int &glog, int &ipl)
     for (int g = 1; g <= 6; ++g) {
         const int lanes = kWave >> g;
         const int p = (c + lanes - 1) / lanes;
-        if (p > kIplMax)
+        if (p > ipl_max)
             continue;
         const float t = (float)p / (float)(1 << g);
         if (t < best * 0.999f) {
@@ -220,9 +221,9 @@ __device__ __forceinline__ void item_shape(int c, int &glog, int &ipl)
 
 // i-chunks of a target with ni particles: k = ceil(ni / 256) near-equal chunks
 // (chunk_model > 0 asks for chunks of at most that many particles instead)
-__device__ __forceinline__ int n_chunks(int ni, float max_chunk)
+__device__ __forceinline__ int n_chunks(int ni, float max_chunk, int ipl_max)
 {
-    int cap = kItemMaxI;
+    int cap = kWave * ipl_max;
     if (max_chunk > 0.f && (int)max_chunk < cap)
         cap = max((int)max_chunk, 1);
     return (ni + cap - 1) / cap;
@@ -244,7 +245,7 @@ __device__ int row_ranges(const RowView &v, int r, long long split, int tgt, int
 {
     if (ni == 0)
         return 0;
-    const int k = n_chunks(ni, v.stage_ops);
+    const int k = n_chunks(ni, v.stage_ops, v.ipl_max);
     const long long wi = (ni + k - 1) / k;
     const int k0 = v.row_off[r], k1 = v.row_off[r + 1];
     int nr = 0, out = out0;
@@ -254,7 +255,7 @@ __device__ int row_ranges(const RowView &v, int r, long long split, int tgt, int
         for (int q = 0; q < k; ++q) {
             int cb, cs, glog, ipl;
             chunk_of(ni, k, q, cb, cs);
-            item_shape(cs, glog, ipl);
+            item_shape(cs, v.ipl_max, glog, ipl);
             items[out] = Item{tgt, ibeg + cb, cs, r, s_beg, s_end, glog, ipl};
             if (cost)
                 cost[out] = (unsigned)min(ns * (long long)((kWave * ipl) >> glog), 0xffffffffll);
@@ -406,7 +407,7 @@ hipError_t rows_cost(const RowArgs &ra, unsigned long long *cost, unsigned long 
     if (temp == nullptr)
         return rocprim::reduce(nullptr, *temp_bytes, cost, total, 0ull, (size_t)ra.nrows,
                                rocprim::plus<unsigned long long>(), st);
-    RowView v{ra.row_tgt, ra.row_off, ra.src, ra.rng, ra.act_cnt, ra.act_off, ra.stage_ops};
+    RowView v{ra.row_tgt, ra.row_off, ra.src, ra.rng, ra.act_cnt, ra.act_off, ra.stage_ops, ra.ipl_max};
     hipLaunchKernelGGL(k_row_cost, dim3(grid_of(ra.nrows)), dim3(256), 0, st, v, ra.nrows, cost);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess)
@@ -421,7 +422,7 @@ hipError_t items_count(const RowArgs &ra, long long split, int *count, int *off,
     if (temp == nullptr)
         return rocprim::exclusive_scan(nullptr, *temp_bytes, count, off, 0, (size_t)ra.nrows + 1,
                                        rocprim::plus<int>(), st);
-    RowView v{ra.row_tgt, ra.row_off, ra.src, ra.rng, ra.act_cnt, ra.act_off, ra.stage_ops};
+    RowView v{ra.row_tgt, ra.row_off, ra.src, ra.rng, ra.act_cnt, ra.act_off, ra.stage_ops, ra.ipl_max};
     hipLaunchKernelGGL(k_items_count, dim3(grid_of(ra.nrows)), dim3(256), 0, st, v, ra.nrows, split, count);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess)
@@ -432,7 +433,7 @@ hipError_t items_count(const RowArgs &ra, long long split, int *count, int *off,
 
 hipError_t items_emit(const RowArgs &ra, long long split, const int *off, Item *items, unsigned *cost, hipStream_t st)
 {
-    RowView v{ra.row_tgt, ra.row_off, ra.src, ra.rng, ra.act_cnt, ra.act_off, ra.stage_ops};
+    RowView v{ra.row_tgt, ra.row_off, ra.src, ra.rng, ra.act_cnt, ra.act_off, ra.stage_ops, ra.ipl_max};
     hipLaunchKernelGGL(k_items_emit, dim3(grid_of(ra.nrows)), dim3(256), 0, st, v, ra.nrows, split, off, items, cost);
     return hipGetLastError();
 }

@@ -213,7 +213,7 @@ int pp2p_set_tuning(pp2p_ctx *c, const pp2p_tuning *t)
     int st = check_ctx(c);
     if (st)
         return st;
-    if (!t || t->waves_per_cu < 0 || t->split_pairs < 0)
+    if (!t || t->waves_per_cu < 0 || t->split_pairs < 0 || t->ipl_max < 0 || t->ipl_max > kIplMax)
         return fail(PP2P_ERR_ARG, "invalid tuning");
     const int segs = t->segs_per_octave;
     if (segs != 0 && segs != 16 && segs != 32)
@@ -487,6 +487,7 @@ static RowArgs row_args(pp2p_ctx *c)
     ra.nrows = c->nrows;
     // chunk_model > 0: at most that many i-particles per item (default 256)
     ra.stage_ops = c->tuning.chunk_model > 0 ? (float)c->tuning.chunk_model : -1.0f;
+    ra.ipl_max = c->tuning.ipl_max >= 1 && c->tuning.ipl_max <= kIplMax ? c->tuning.ipl_max : kIplMax;
     return ra;
 }
 

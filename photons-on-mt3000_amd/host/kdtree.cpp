@@ -16,9 +16,10 @@
 // 40-byte Body records; the partition moves them identically.
 //
 // Differences from the reference, by design: leaf and node ids start at 0
-// (the reference offsets them by first_leaf = NPART and first_node); the walk
-// runs its top levels in parallel (OpenMP tasks) and concatenates the per-task
-// outputs in depth-first order, which reproduces the sequential order exactly.
+// (the reference offsets them by first_leaf = NPART and first_node); the build
+// and the walk run their top levels in parallel (OpenMP tasks) and concatenate
+// the per-task outputs in depth-first order, which reproduces the sequential
+// numbering and order exactly.
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -103,30 +104,77 @@ void mean_split(Tree &t, int D, long long p, long long len, long long np[2], dou
 }
 
 // Depth-first build; ids in creation order (leaves and nodes separately).
-void build(Tree &t, int direct, long long ip, long long len, int inode)
+// A subtree is built into its own arena with local ids (its root is node 0,
+// leaves counted from 0); big subtrees are built by parallel OpenMP tasks and
+// the arenas are concatenated in depth-first order, which yields exactly the
+// ids of the sequential build.
+struct Arena {
+    std::vector<Node> node;
+    std::vector<Leaf> leaf;
+};
+
+void append_arena(Arena &dst, Arena &src, int &son_ref)
 {
-    if (len == 0)
-        return;
+    const int node_shift = (int)dst.node.size();
+    const int leaf_shift = (int)dst.leaf.size();
+    for (Node &b : src.node)
+        for (int n = 0; n < 2; ++n)
+            b.son[n] = b.son[n] >= 0 ? b.son[n] + node_shift : b.son[n] - leaf_shift;
+    son_ref = node_shift;
+    dst.node.insert(dst.node.end(), src.node.begin(), src.node.end());
+    dst.leaf.insert(dst.leaf.end(), src.leaf.begin(), src.leaf.end());
+    Arena().node.swap(src.node);
+    std::vector<Leaf>().swap(src.leaf);
+}
+
+void build_arena(Tree &t, int direct, long long ip, long long len, Arena &out, int depth)
+{
+    out.node.push_back(Node{{-1, -1}, 0.f, {0, 0, 0}, {0, 0, 0}});
     long long np[2];
     double split = 0.0;
     mean_split(t, direct, ip, len, np, &split);
-    t.node[inode].split = (float)split;
+    out.node[0].split = (float)split;
     const int nd = (direct + 1) % 3;
+    const long long ipc[2] = {ip, ip + np[0]};
+    const bool is_node[2] = {np[0] > t.maxleaf, np[1] > t.maxleaf};
+    Arena child[2];
+    const bool par = depth < 12 && len > 200000;
     for (int n = 0; n < 2; ++n) {
-        if (np[n] <= t.maxleaf) {
-            Leaf l{};
-            l.ipart = (int)ip;
-            l.npart = (int)np[n];
-            t.node[inode].son[n] = -(int)t.leaf.size() - 1;
-            t.leaf.push_back(l);
+        if (!is_node[n])
+            continue;
+        if (par) {
+#pragma omp task default(shared) firstprivate(n)
+            build_arena(t, nd, ipc[n], np[n], child[n], depth + 1);
         } else {
-            const int id = (int)t.node.size();
-            t.node.push_back(Node{{-1, -1}, 0.f, {0, 0, 0}, {0, 0, 0}});
-            t.node[inode].son[n] = id;
-            build(t, nd, ip, np[n], id);
+            build_arena(t, nd, ipc[n], np[n], child[n], depth + 1);
         }
-        ip += np[n];
     }
+    if (par) {
+#pragma omp taskwait
+    }
+    for (int n = 0; n < 2; ++n) {
+        if (is_node[n]) {
+            int ref;
+            append_arena(out, child[n], ref);
+            out.node[0].son[n] = ref;
+        } else {
+            Leaf l{};
+            l.ipart = (int)ipc[n];
+            l.npart = (int)np[n];
+            out.node[0].son[n] = -(int)out.leaf.size() - 1;
+            out.leaf.push_back(l);
+        }
+    }
+}
+
+void build(Tree &t, int direct, long long ip, long long len, int nthreads)
+{
+    Arena root;
+#pragma omp parallel num_threads(nthreads)
+#pragma omp single
+    build_arena(t, direct, ip, len, root, 0);
+    t.node.swap(root.node);
+    t.leaf.swap(root.leaf);
 }
 
 // Boxes from the split planes (src/fmm.c:273-327), FP32.
@@ -366,10 +414,7 @@ phost_lists *phost_build(long long n, const int *posi, const int *tag, double in
             t.x[d][i] = posi[3 * i + d];
         t.idx[i] = (int)i;
     }
-    t.node.reserve(2 * (n / std::max(1, maxleaf / 2)) + 16);
-    t.leaf.reserve(2 * (n / std::max(1, maxleaf / 2)) + 16);
-    t.node.push_back(Node{{-1, -1}, 0.f, {0, 0, 0}, {0, 0, 0}});
-    build(t, 0, 0, n, 0);
+    build(t, 0, 0, n, nthreads);
     float left[3] = {bdl[0], bdl[1], bdl[2]}, right[3] = {bdr[0], bdr[1], bdr[2]};
     boxes(t, 0, 0, left, right);
     // targets: leaves holding a local particle (src/fmm.c:1226-1236)

@@ -51,7 +51,7 @@ class _Params(ctypes.Structure):
 class _Tuning(ctypes.Structure):
     _fields_ = [("waves_per_cu", ctypes.c_int), ("split_pairs", ctypes.c_int),
                 ("segs_per_octave", ctypes.c_int), ("no_cull", ctypes.c_int),
-                ("chunk_model", ctypes.c_int), ("reserved", ctypes.c_int * 3)]
+                ("chunk_model", ctypes.c_int), ("ipl_max", ctypes.c_int), ("reserved", ctypes.c_int * 2)]
 
 
 # Every symbol include/photons_p2p.h declares.
@@ -181,15 +181,16 @@ class Context:
         self._chk(self._lib.pp2p_set_params(self._h, ctypes.byref(p)))
 
     def set_tuning(self, waves_per_cu: int = 0, split_pairs: int = 0, segs_per_octave: int = 0,
-                   no_cull: int = 0, chunk_model: int = 0):
-        t = _Tuning(waves_per_cu, split_pairs, segs_per_octave, no_cull, chunk_model)
+                   no_cull: int = 0, chunk_model: int = 0, ipl_max: int = 0):
+        t = _Tuning(waves_per_cu, split_pairs, segs_per_octave, no_cull, chunk_model, ipl_max)
         self._chk(self._lib.pp2p_set_tuning(self._h, ctypes.byref(t)))
 
     def get_tuning(self) -> dict:
         t = _Tuning()
         self._chk(self._lib.pp2p_get_tuning(self._h, ctypes.byref(t)))
         return {"waves_per_cu": t.waves_per_cu, "split_pairs": t.split_pairs,
-                "segs_per_octave": t.segs_per_octave, "no_cull": t.no_cull, "chunk_model": t.chunk_model}
+                "segs_per_octave": t.segs_per_octave, "no_cull": t.no_cull, "chunk_model": t.chunk_model,
+                "ipl_max": t.ipl_max}
 
     # -- data ----------------------------------------------------------------
     def set_particles(self, pos: np.ndarray, leaf_ip: np.ndarray, leaf_np: np.ndarray):

@@ -1,0 +1,107 @@
+"""Autotuner of the P2P kernel's launch and work-item shape.
+
+Times candidate tunings of one prepared list on the device and keeps the
+fastest.  The knobs are the ones pp2p_tuning exposes:
+  ipl_max      i-particles per lane an item may use (1..4); with it the item
+               builder picks G phases x IPL slots per item
+  split_pairs  size of a queue item (pair-interactions) = the j-range a wave
+               owns before it fetches the next item
+  waves_per_cu persistent waves per CU of the queue form (0 = occupancy max)
+  law          LOGTAB table or the table-free POLY law
+Tuned values are kept in photons_p2p/tuned.json, keyed by a signature of
+the workload shape (particle, leaf, task and pair counts in half-octave
+buckets); `lookup` returns the entry for a shape, `tune` measures a new one.
+"""
+from __future__ import annotations
+
+import json
+import os
+import time
+
+from . import Context, EXEC_QUEUE, ForceLaw, LAW_POLY, LAW_TABLE
+
+TUNED = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned.json")
+
+GRID = {
+    "ipl_max": [1, 2, 3, 4],
+    "split_pairs": [0, 1 << 16, 1 << 18, 1 << 20],
+    "waves_per_cu": [0, 12, 8],
+    "law": [LAW_TABLE, LAW_POLY],
+}
+
+
+def signature(ntask: int, npart: int, nleaf: int, npairs: int) -> str:
+    b = lambda x: int(round(2 * __import__("math").log2(max(x, 1))))  # noqa: E731
+    return f"p{b(npart)}-l{b(nleaf)}-t{b(ntask)}-n{b(npairs)}"
+
+
+def _time(ctx: Context, steps: int) -> float:
+    ctx.time_steps(EXEC_QUEUE, 1)                       # warm (builds the queue for this tuning)
+    tot, _ = ctx.time_steps(EXEC_QUEUE, steps)
+    return tot / steps
+
+
+def tune(ctx: Context, law: ForceLaw, steps: int = 3, grid: dict | None = None, log=None,
+         budget_s: float = 120.0) -> dict:
+    """Coordinate search over `grid` (one knob at a time, in the order above,
+    starting from the defaults).  The context must hold particles and a list.
+    Returns {"best": tuning, "ms": ms_per_step, "trials": [...]}; the context
+    is left with the best tuning applied."""
+    g = dict(GRID if grid is None else grid)
+    best = {"ipl_max": 0, "split_pairs": 0, "waves_per_cu": 0, "law": law.law}
+    trials = []
+    t_start = time.time()
+
+    def set_cfg(cfg):
+        apply(ctx, law, cfg)
+
+    set_cfg(best)
+    best_ms = _time(ctx, steps)
+    trials.append({**best, "ms": best_ms})
+    for knob in ("ipl_max", "split_pairs", "waves_per_cu", "law"):
+        for v in g.get(knob, []):
+            if v == best[knob] or time.time() - t_start > budget_s:
+                continue
+            cfg = {**best, knob: v}
+            set_cfg(cfg)
+            ms = _time(ctx, steps)
+            trials.append({**cfg, "ms": ms})
+            if log:
+                log(f"[autotune] {cfg} -> {ms:.3f} ms")
+            if ms < best_ms:
+                best, best_ms = cfg, ms
+    set_cfg(best)
+    return {"best": best, "ms": best_ms, "trials": trials}
+
+
+def load_table(path: str = TUNED) -> dict:
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def save_table(entries: dict, path: str):
+    d = os.path.dirname(os.path.abspath(path))
+    os.makedirs(d, exist_ok=True)
+    tmp = path + ".tmp"
+    with open(tmp, "w") as f:
+        json.dump(entries, f, indent=1, sort_keys=True)
+    os.replace(tmp, path)
+
+
+def lookup(sig: str, path: str = TUNED) -> dict | None:
+    """Tuned knobs for a workload shape, or None."""
+    e = load_table(path).get(sig)
+    return None if e is None else e.get("best")
+
+
+def apply(ctx: Context, law: ForceLaw, cfg: dict):
+    ctx.set_law(ForceLaw(eps=law.eps, rs=law.rs, mass=law.mass, ucut=law.ucut, softening=law.softening,
+                         law=cfg.get("law", law.law), frame=law.frame, box=law.box))
+    ctx.set_tuning(waves_per_cu=cfg.get("waves_per_cu", 0), split_pairs=cfg.get("split_pairs", 0),
+                   ipl_max=cfg.get("ipl_max", 0))
+
+
+__all__ = ["tune", "lookup", "apply", "signature", "GRID", "load_table", "save_table"]

@@ -402,3 +402,32 @@ def config_precision(n1: int = 128, box: float = 100.0, seed: int = 20261204, n_
                       meta={"seed": seed, "n1": n1})
     w.meta["offset"] = offset_boxes * box
     return w
+
+
+def active_subset(w: Workload, fraction: float, seed: int = 20261207, level: int = 1,
+                  n_grid: int | None = None) -> tuple[np.ndarray, np.ndarray]:
+    """Hierarchical-time-step active mask of a tree workload: the first
+    `fraction` of the local particles in a seeded order that puts members of
+    Plummer spheres (input index >= n_grid, clustered_mix order) first get
+    act = level, all others (and every ghost copy) act = 0.  Returns (act,
+    task mask): the tasks whose target leaf holds an active particle, the
+    targets the reference's active walk emits (src/fmm.c:1308-1320, :1046).
+    Without n_grid the order is a plain seeded permutation."""
+    m = w.meta
+    tag = m["tag"]
+    local = m["local_index"]
+    nlocal = int(m["nlocal"])
+    g = rng_for(seed)
+    order = g.permutation(nlocal)
+    if n_grid is not None:
+        member = order >= n_grid
+        order = np.concatenate([order[member], order[~member]])
+    nact = int(round(fraction * nlocal))
+    chosen = np.zeros(nlocal, bool)
+    chosen[order[:nact]] = True
+    act = np.where((tag == 1) & chosen[local], level, 0).astype(np.uint8)
+    # targets: leaves holding an active particle
+    leaf_of = np.repeat(np.arange(w.nleaf), w.leaf_np)
+    has = np.zeros(w.nleaf, bool)
+    has[leaf_of[act == level]] = True
+    return act, has[w.task_t]

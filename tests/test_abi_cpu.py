@@ -100,3 +100,14 @@ def test_law_parameter_validation():
         pp.law_eval(pp.ForceLaw(eps=10.0, rs=1.0), np.ones(1, np.float32))     # eps beyond the cutoff
     with pytest.raises(pp.P2PError):
         pp.law_eval(pp.ForceLaw(eps=0.1, rs=1.0), np.ones(1, np.float32), segs_per_octave=7)
+
+
+def test_autotune_table_roundtrip(tmp_path):
+    from photons_p2p import autotune as AT
+    sig = AT.signature(1000, 5000, 300, 10 ** 9)
+    assert sig == AT.signature(1010, 5050, 301, 10 ** 9 + 5)        # same half-octave buckets
+    assert sig != AT.signature(4000, 5000, 300, 10 ** 9)
+    p = str(tmp_path / "t.json")
+    AT.save_table({sig: {"best": {"ipl_max": 3, "split_pairs": 0, "waves_per_cu": 0, "law": 0}, "ms": 1.0}}, p)
+    assert AT.lookup(sig, p)["ipl_max"] == 3
+    assert AT.lookup("nope", p) is None

@@ -150,3 +150,32 @@ def test_streamed_lists_accumulate(gpu_ctx):
     ctx.compute()
     a2 = ctx.acc()
     np.testing.assert_allclose(a2, 2 * a, rtol=1e-12, atol=1e-300)
+
+
+def test_autotune_keeps_results_and_picks_a_measured_config(gpu_ctx):
+    from photons_p2p import autotune as AT
+    w = _small_uneven(seed=19)
+    ref = _oracle(w)
+    law = pp.ForceLaw(eps=w.eps, rs=w.rs, softening=w.softening, box=w.box)
+    gpu_ctx.set_law(law)
+    gpu_ctx.set_tuning()
+    gpu_ctx.set_particles(w.pos, w.leaf_ip, w.leaf_np)
+    gpu_ctx.set_tasks(w.task_t, w.task_s)
+    res = AT.tune(gpu_ctx, law, steps=1, grid={"ipl_max": [1, 2], "split_pairs": [1 << 14], "waves_per_cu": [8],
+                                               "law": [pp.LAW_POLY]})
+    assert res["ms"] == min(t["ms"] for t in res["trials"])
+    gpu_ctx.zero_acc()
+    gpu_ctx.compute()
+    med, p99, mx = pyoracle.rel_err(gpu_ctx.acc(), ref)
+    assert p99 <= P99_TOL and mx <= MAX_TOL
+    gpu_ctx.set_tuning()
+
+
+@pytest.mark.parametrize("ipl_max", [1, 2, 3])
+def test_ipl_caps(gpu_ctx, ipl_max):
+    w = _small_uneven(seed=20)
+    ref = _oracle(w)
+    a = _run(gpu_ctx, w, ipl_max=ipl_max)
+    med, p99, mx = pyoracle.rel_err(a, ref)
+    assert p99 <= P99_TOL and mx <= MAX_TOL
+    gpu_ctx.set_tuning()
