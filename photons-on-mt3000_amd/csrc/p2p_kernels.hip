@@ -26,6 +26,25 @@ __device__ __forceinline__ void wave_lds_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
+// Per-wave LDS directory of up to 64 source leaves (see process_item).
+struct SrcDir {
+    int pre[kWave];       // inclusive prefix of the source sizes
+    int base[kWave];      // pos index minus window position of each source's first particle
+    double ox[kWave], oy[kWave], oz[kWave];   // FP64 frame offsets (target frame - source frame)
+};
+
+// inclusive prefix sum over the wave (lane order)
+__device__ __forceinline__ int wave_incl_scan(int v, int lane)
+{
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const int u = __shfl_up(v, o, kWave);
+        if (lane >= o)
+            v += u;
+    }
+    return v;
+}
+
 // lane k's double, as a wave-uniform value
 __device__ __forceinline__ double rl_double(double v, int k)
 {
@@ -130,8 +149,8 @@ __device__ __forceinline__ float4 null_particle() { return make_float4(1e18f, 1e
 // i-particle meet by a butterfly over lanes.
 template <int IPL, int LAW, int SOFT, bool FOLD64, bool INDIRECT, bool COUNT>
 __device__ __forceinline__ void process_item(const ForceArgs &a, const LawConst &lc, const Item &it,
-                                             const float4 *__restrict__ s_tab, float4 *tile, int lane,
-                                             unsigned long long &n_in_total)
+                                             const float4 *__restrict__ s_tab, float4 *tile, SrcDir &dir,
+                                             int lane, unsigned long long &n_in_total)
 {
     const int ni = it.ni;
     const int glog = it.pad0;
@@ -207,9 +226,11 @@ __device__ __forceinline__ void process_item(const ForceArgs &a, const LawConst 
         }
     };
 
-    // Source metadata 64 sources at a time (one per lane, read back with
-    // readlane); particle chunks of 64 prefetched one chunk ahead so the
-    // global-load latency overlaps the pair loop of the current tile.
+    // Sources 64 at a time: lane s loads source s's range and FP64 frame
+    // offset into the wave's LDS directory with the inclusive prefix of the
+    // source sizes.  Particles are then staged in windows of 64 that run over
+    // the concatenated sources (a lane finds its source by binary search in
+    // the prefix), the next window prefetched while the current tile computes.
     for (int sb = it.s_beg; sb < it.s_end; sb += kWave) {
         const int nsrc = min(kWave, it.s_end - sb);
         int m_ip = 0, m_np = 0;
@@ -229,35 +250,45 @@ __device__ __forceinline__ void process_item(const ForceArgs &a, const LawConst 
                 }
             }
         }
-        int k = 0, c0 = 0;
-        int ip_k = __builtin_amdgcn_readlane(m_ip, 0);
-        int np_k = __builtin_amdgcn_readlane(m_np, 0);
+        const int pre = wave_incl_scan(m_np, lane);          // sources 0..lane
+        const int total = __builtin_amdgcn_readlane(pre, kWave - 1);
+        dir.pre[lane] = pre;
+        dir.base[lane] = m_ip - (pre - m_np);               // pos index of window position 0 of source lane
+        dir.ox[lane] = m_ox;
+        dir.oy[lane] = m_oy;
+        dir.oz[lane] = m_oz;
+        wave_lds_sync();
+
+        // window w covers concatenated positions [w, w + 64)
+        auto locate = [&](int J) -> int {                   // smallest s with pre[s] > J
+            int lo = 0;
+#pragma unroll
+            for (int step = 32; step > 0; step >>= 1)
+                if (dir.pre[lo + step - 1] <= J)
+                    lo += step;
+            return lo;
+        };
+        int J = lane;
+        int ks = J < total ? locate(J) : 0;
         float4 qv = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (lane < np_k)
-            qv = a.pos[ip_k + lane];
-        while (k < nsrc) {
-            // next chunk position and its prefetch
-            int k2 = k, c2 = c0 + kWave;
-            if (c2 >= np_k) {
-                k2 = k + 1;
-                c2 = 0;
-            }
-            int ip2 = 0, np2 = 0;
+        if (J < total)
+            qv = a.pos[dir.base[ks] + J];
+        for (int w0 = 0; w0 < total; w0 += kWave) {
+            // prefetch the next window
+            const int J2 = J + kWave;
+            int ks2 = 0;
             float4 q2 = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (k2 < nsrc) {
-                ip2 = __builtin_amdgcn_readlane(m_ip, k2);
-                np2 = __builtin_amdgcn_readlane(m_np, k2);
-                if (lane < np2 - c2)
-                    q2 = a.pos[ip2 + c2 + lane];
+            if (J2 < total) {
+                ks2 = locate(J2);
+                q2 = a.pos[dir.base[ks2] + J2];
             }
-            // stage the current chunk: target frame, cull, compact
-            const bool has = lane < np_k - c0;
+            // stage the current window: target frame, cull, compact
+            const bool has = J < total;
             float4 w = qv;
-            if (FOLD64) {
-                const double ox = rl_double(m_ox, k), oy = rl_double(m_oy, k), oz = rl_double(m_oz, k);
-                w.x = (float)((double)qv.x + ox);
-                w.y = (float)((double)qv.y + oy);
-                w.z = (float)((double)qv.z + oz);
+            if (FOLD64 && has) {
+                w.x = (float)((double)qv.x + dir.ox[ks]);
+                w.y = (float)((double)qv.y + dir.oy[ks]);
+                w.z = (float)((double)qv.z + dir.oz[ks]);
             }
             const float ex = fmaxf(fabsf(w.x - cx) - hx, 0.f);
             const float ey = fmaxf(fabsf(w.y - cy) - hy, 0.f);
@@ -282,13 +313,10 @@ __device__ __forceinline__ void process_item(const ForceArgs &a, const LawConst 
                 wave_lds_sync();
             }
             qv = q2;
-            if (k2 != k) {
-                ip_k = ip2;
-                np_k = np2;
-            }
-            k = k2;
-            c0 = c2;
+            ks = ks2;
+            J = J2;
         }
+        wave_lds_sync();                               // directory reused by the next batch
     }
     if (cnt > 0) {
         const int n = ((cnt + G - 1) >> glog) << glog;
@@ -330,6 +358,7 @@ void p2p_force(ForceArgs a, LawConst lc)
 {
     extern __shared__ float4 s_tab[];                       // LOGTAB table
     __shared__ float4 s_tile[kWavesPerGroup][kTileCap];     // j-tiles, one per wave
+    __shared__ SrcDir s_dir[kWavesPerGroup];                // source directories, one per wave
 
     const int wv = threadIdx.x / kWave;
     const int lane = threadIdx.x % kWave;
@@ -355,10 +384,10 @@ void p2p_force(ForceArgs a, LawConst lc)
             break;
         const Item it = a.items[item_id];
         switch (it.pad1) {
-        case 1: process_item<1, LAW, SOFT, FOLD64, INDIRECT, COUNT>(a, lc, it, s_tab, tile, lane, n_in); break;
-        case 2: process_item<2, LAW, SOFT, FOLD64, INDIRECT, COUNT>(a, lc, it, s_tab, tile, lane, n_in); break;
-        case 3: process_item<3, LAW, SOFT, FOLD64, INDIRECT, COUNT>(a, lc, it, s_tab, tile, lane, n_in); break;
-        default: process_item<4, LAW, SOFT, FOLD64, INDIRECT, COUNT>(a, lc, it, s_tab, tile, lane, n_in); break;
+        case 1: process_item<1, LAW, SOFT, FOLD64, INDIRECT, COUNT>(a, lc, it, s_tab, tile, s_dir[wv], lane, n_in); break;
+        case 2: process_item<2, LAW, SOFT, FOLD64, INDIRECT, COUNT>(a, lc, it, s_tab, tile, s_dir[wv], lane, n_in); break;
+        case 3: process_item<3, LAW, SOFT, FOLD64, INDIRECT, COUNT>(a, lc, it, s_tab, tile, s_dir[wv], lane, n_in); break;
+        default: process_item<4, LAW, SOFT, FOLD64, INDIRECT, COUNT>(a, lc, it, s_tab, tile, s_dir[wv], lane, n_in); break;
         }
         if (!QUEUE)
             break;
