@@ -43,15 +43,15 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make_workload(name: str, n1: int | None, threads: int):
+def make_workload(name: str, n1: int | None, threads: int, softening: int = 1):
     from photons_p2p import workloads as W
     if name == "clustered":
         n = n1 or 256
         # 256^3 in 400 Mpc/h; 512^3 in 1 Gpc/h (the large single-GPU share)
         box = 1000.0 if n == 512 else 400.0 * n / 256
-        return W.config_clustered(n, box=box, nthreads=threads)
+        return W.config_clustered(n, box=box, nthreads=threads, progress=lambda m: log(f"[bench] {m}"))
     if name == "plain":
-        return W.config_plain(n1 or 64, nthreads=threads)
+        return W.config_plain(n1 or 64, nthreads=threads, softening=softening)
     if name == "precision":
         return W.config_precision(n1 or 128, nthreads=threads)
     if name == "uneven":
@@ -80,14 +80,26 @@ def accuracy(w, acc, mask, threads, pos=None, act=None, level=1):
     return pyoracle.rel_err(acc[idx], ref[idx]), idx.size
 
 
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(w, threads: int, budget_s: float):
-    """FP32 restatement of the reference kernel on a bounded row sample."""
+    """FP32 restatement of the reference kernel on a bounded row sample:
+    the sample grows until one pass takes about a third of the budget, then
+    runs twice more; the best of the three passes counts."""
     import pyoracle
     prm = pyoracle.params(w.eps, w.rs, mass=w.mass, softening=w.softening, box=w.box)
     pos32 = np.ascontiguousarray(w.pos.T.astype(np.float32))
     ntgt = np.unique(w.task_t).size
     nrows = 16
-    rate = None
     while True:
         mask = sample_rows(w, nrows, seed=99)
         csr = pyoracle.build_csr(w.task_t[mask], w.task_s[mask], w.nleaf)
@@ -95,10 +107,45 @@ def cpu_baseline(w, threads: int, budget_s: float):
         t0 = time.perf_counter()
         pyoracle.p2p_ref32(prm, pos32, w.leaf_ip, w.leaf_np, csr, nthreads=threads)
         dt = time.perf_counter() - t0
-        rate = npairs / dt
-        if dt >= 0.5 * budget_s or nrows >= ntgt:
-            return rate, nrows, npairs, dt
-        nrows = min(ntgt, max(2 * nrows, int(nrows * budget_s / max(dt, 1e-3))))
+        if dt >= budget_s / 6 or nrows >= ntgt:
+            break
+        nrows = min(ntgt, max(2 * nrows, int(nrows * budget_s / 3 / max(dt, 1e-3))))
+    best = dt
+    for _ in range(2):
+        t0 = time.perf_counter()
+        pyoracle.p2p_ref32(prm, pos32, w.leaf_ip, w.leaf_np, csr, nthreads=threads)
+        best = min(best, time.perf_counter() - t0)
+    return npairs / best, nrows, npairs, best
+
+
+def copy_calibration(mib: int) -> dict:
+    """Five device-to-device copies of `mib` MiB (hipMemcpy's copy kernel): a
+    kernel whose HBM traffic is known exactly (mib MiB read + mib MiB written
+    per launch), profiled beside the force kernel so that the FETCH_SIZE /
+    WRITE_SIZE counters can be scaled to bytes."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = ctypes.c_size_t(mib << 20)
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
+    for rc in (hip.hipMalloc(ctypes.byref(a), n), hip.hipMalloc(ctypes.byref(b), n),
+               hip.hipMemset(a, 1, n), hip.hipEventCreate(ctypes.byref(e0)), hip.hipEventCreate(ctypes.byref(e1))):
+        if rc != 0:
+            raise RuntimeError(f"copy calibration: HIP error {rc}")
+    t = []
+    for _ in range(5):
+        hip.hipEventRecord(e0, None)
+        hip.hipMemcpyAsync(b, a, n, 3, None)                # 3 = hipMemcpyDeviceToDevice
+        hip.hipEventRecord(e1, None)
+        hip.hipEventSynchronize(e1)
+        ms = ctypes.c_float()
+        hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1)
+        t.append(ms.value)
+    hip.hipFree(a)
+    hip.hipFree(b)
+    ms = float(np.median(t))
+    log(f"[bench] copy calibration: {mib} MiB in {ms:.3f} ms ({2 * (mib << 20) / ms / 1e6:.0f} GB/s)")
+    return {"mib": mib, "ms": ms, "bytes_read": mib << 20, "bytes_written": mib << 20}
 
 
 def main():
@@ -108,6 +155,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="clustered", choices=["clustered", "plain", "precision", "uneven"])
     ap.add_argument("--n1", type=int, default=None, help="particles per side (default: the config's)")
+    ap.add_argument("--softening", default="plummer", choices=["plummer", "ref"],
+                    help="plain workload: Plummer (eps = spacing / 40) or the reference's max(r, eps) rule "
+                         "(eps = SoftenScale = 0.03 cell)")
     ap.add_argument("--exec", default="queue", choices=["queue", "grid", "per_target"])
     ap.add_argument("--law", default="table", choices=["table", "poly"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -127,6 +177,12 @@ def main():
                     help="active fraction (0: every particle); targets with an active particle only")
     ap.add_argument("--precision-forms", action="store_true",
                     help="precision workload: also evaluate FP32 absolute coordinates and report both errors")
+    ap.add_argument("--compare-grid", action="store_true",
+                    help="also time one launch with one wave per item in list order (no persistence)")
+    ap.add_argument("--active-list", default=None,
+                    help="comma-separated active fractions evaluated one after another on the same workload")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="after the run, copy a buffer of this many MiB 5 times (HBM counter calibration)")
     ap.add_argument("--compare-plain", action="store_true",
                     help="also time the one-launch-per-target form the same way and report the ratio")
     ap.add_argument("--chunk-model", type=int, default=0, help="i-chunking: 0 default, -1 chunks of 64, >0 cost")
@@ -140,7 +196,7 @@ def main():
     import photons_p2p as pp
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
     t0 = time.time()
-    w = make_workload(args.workload, args.n1, threads)
+    w = make_workload(args.workload, args.n1, threads, softening=0 if args.softening == "ref" else 1)
     log(f"[bench] workload {w.name}: N={w.npart} leaves={w.nleaf} tasks={w.task_t.size} "
         f"built in {time.time() - t0:.1f}s")
     act = None
@@ -217,6 +273,12 @@ def main():
         plain = {"ms_per_step": p_ms, "value": n_list / (p_ms * 1e-3) / 1e9, "speedup_of_queue": p_ms / ms_step,
                  "form": "one launch per target leaf", "graph_steps": args.graph}
         log(f"[bench] per-target form: {p_ms:.3f} ms/step; queue form is {p_ms / ms_step:.2f}x faster")
+    grid_form = None
+    if args.compare_grid:
+        g_ms, _, _ = timed(pp.EXEC_GRID)
+        grid_form = {"ms_per_step": g_ms, "value": n_list / (g_ms * 1e-3) / 1e9, "speedup_of_queue": g_ms / ms_step,
+                     "form": "one launch, one wave per item, list order", "graph_steps": args.graph}
+        log(f"[bench] grid form: {g_ms:.3f} ms/step; queue form is {g_ms / ms_step:.2f}x faster")
     value = n_list / (ms_step * 1e-3) / 1e9
     flops = FLOPS_IN * n_in + FLOPS_OUT * (n_list - n_in)
     achieved = flops / t_kernel / 1e12
@@ -247,14 +309,48 @@ def main():
                      "fp32_absolute": {"median": m2, "p99": p2, "max": x2},
                      "oracle": "FP64 direct sum, same lists and force law"}
             log(f"[bench] FP32 absolute coordinates: median {m2:.2e} p99 {p2:.2e} max {x2:.2e}")
+    sweep = None
+    if args.active_list:
+        from photons_p2p import workloads as W
+        n_grid = (w.meta["n1"] ** 3) // 2 if w.name.startswith("clustered") else None
+        sweep = []
+        for f in [float(x) for x in args.active_list.split(",")]:
+            a_f, tm = W.active_subset(w, f, n_grid=n_grid)
+            ctx.set_law(pp.ForceLaw(eps=w.eps, rs=w.rs, mass=w.mass, softening=w.softening, law=law, box=w.box))
+            ctx.set_particles(pos_eval, w.leaf_ip, w.leaf_np)
+            ctx.set_active(a_f, 1)
+            ctx.set_tasks(w.task_t[tm], w.task_s[tm])
+            p_ms = ctx.prepare()
+            sf = ctx.list_stats()
+            f_ms, f_force, _ = timed(exec_form)
+            nact = int((a_f == 1).sum())
+            ent = {"fraction": f, "active_particles": nact, "ntask": int(tm.sum()), "n_list": int(sf["npairs"]),
+                   "ms_per_step": f_ms, "force_ms": f_force, "prepare_ms": p_ms,
+                   "active_per_s": nact / (f_ms * 1e-3), "ms_vs_all_active": f_ms / ms_step}
+            if not args.no_acc:
+                sub = w.task_t[tm]
+                tg = np.unique(sub)
+                pick = np.random.default_rng(7).choice(tg, size=min(100, tg.size), replace=False)
+                mk = tm & np.isin(w.task_t, pick)
+                (m3, p3, x3), nck = accuracy(w, ctx.acc(), mk, threads, pos=pos_eval, act=a_f)
+                ent["rel_err"] = {"median": m3, "p99": p3, "max": x3, "particles": int(nck)}
+            sweep.append(ent)
+            log(f"[bench] active {f}: {nact} active, {ent['ntask']} tasks, N_list {sf['npairs']:.3e}, "
+                f"{f_ms:.3f} ms/step ({f_ms / ms_step:.3f} of the all-active step), prepare {p_ms:.1f} ms"
+                + (f", p99 {ent['rel_err']['p99']:.2e}" if "rel_err" in ent else ""))
     ctx.close()
+
+    copy_cal = None
+    if args.copy_calibration:
+        copy_cal = copy_calibration(args.copy_calibration)
 
     cpu = None
     if not args.no_cpu:
         rate, nrows, npairs, dt = cpu_baseline(w, threads, args.cpu_budget)
         cpu = {"value": rate / 1e9, "unit": "Gpairs/s", "cores": threads, "kind": "port",
                "sample": f"{nrows} target leaves ({npairs:.3e} pair-interactions) of the same list, "
-                         f"FP32 restatement of P2P_kernel/P2P_core, {dt:.1f}s"}
+                         f"FP32 restatement of P2P_kernel/P2P_core, best of 3 passes ({dt:.2f}s)",
+               "cpu_model": cpu_model(), "nproc": os.cpu_count()}
         log(f"[bench] cpu baseline: {rate / 1e9:.3f} Gpairs/s on {threads} threads ({dt:.1f}s)")
 
     out = {
@@ -282,6 +378,9 @@ def main():
         "active_fraction": args.active if args.active > 0 else None,
         "precision_forms": forms,
         "plain_form": plain,
+        "grid_form": grid_form,
+        "active_sweep": sweep,
+        "copy_calibration": copy_cal,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None},
         "cpu_baseline": cpu,

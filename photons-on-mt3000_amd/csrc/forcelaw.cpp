@@ -91,6 +91,7 @@ bool law_setup(int softening, double eps, double rs, double ucut, int segs, LawS
         ++lg;
     ls.lc.seg_shift = 23 - lg;
     ls.lc.seg_bits = 5 + lg;
+    ls.lc.tab_shift = ls.lc.seg_shift - 4;
     ls.lc.k2 = (float)(-ucut * ucut / std::log(2.0));
 
     // LOGTAB: entry index = (biased exponent mod 32) * segs + top mantissa bits

@@ -195,25 +195,25 @@ __global__ void k_row_cost(RowView v, int nrows, unsigned long long *__restrict_
     cost[r] = ns * (unsigned long long)ni;
 }
 
-// Shape of an item with c i-particles: G = 2^glog phases (lane groups that
-// split the sources), IPL i-particles per lane, 64/G * IPL >= c slots.  The
-// wave time per source particle is proportional to IPL / G; the smallest value
-// wins, ties go to fewer phases.  48 -> G 4 x IPL 3 (all lanes busy),
-// 24 -> 8 x 3, 64 -> 1 x 1, 1 -> 64 x 1.
-__device__ __forceinline__ void item_shape(int c, int ipl_max, int &glog, int &ipl)
+// Shape of an item with c i-particles: L lanes per phase, G = 64 / L phases
+// (lane groups that split the sources; lanes beyond G * L idle), IPL
+// i-particles per lane, L * IPL >= c slots.  The wave time per source particle
+// is proportional to IPL / G; the smallest value wins, ties go to fewer
+// phases.  48 -> L 16 (G 4) x IPL 3, 40 -> L 20 (G 3) x IPL 2, 64 -> L 64 x 1,
+// 1 -> L 1 (G 64) x 1.
+__device__ __forceinline__ void item_shape(int c, int ipl_max, int &lanes, int &ipl)
 {
-    glog = 0;
+    lanes = kWave;
     ipl = (c + kWave - 1) / kWave;
     float best = (float)ipl;
-    for (int g = 1; g <= 6; ++g) {
-        const int lanes = kWave >> g;
-        const int p = (c + lanes - 1) / lanes;
+    for (int l = kWave - 1; l >= 1; --l) {
+        const int p = (c + l - 1) / l;
         if (p > ipl_max)
-            continue;
-        const float t = (float)p / (float)(1 << g);
+            break;                                   // fewer lanes only need more slots
+        const float t = (float)p / (float)(kWave / l);
         if (t < best * 0.999f) {
             best = t;
-            glog = g;
+            lanes = l;
             ipl = p;
         }
     }
@@ -253,12 +253,12 @@ __device__ int row_ranges(const RowView &v, int r, long long split, int tgt, int
     long long nsum = 0;
     auto emit = [&](int s_beg, int s_end, long long ns) {
         for (int q = 0; q < k; ++q) {
-            int cb, cs, glog, ipl;
+            int cb, cs, lanes, ipl;
             chunk_of(ni, k, q, cb, cs);
-            item_shape(cs, v.ipl_max, glog, ipl);
-            items[out] = Item{tgt, ibeg + cb, cs, r, s_beg, s_end, glog, ipl};
+            item_shape(cs, v.ipl_max, lanes, ipl);
+            items[out] = Item{tgt, ibeg + cb, cs, r, s_beg, s_end, lanes, ipl};
             if (cost)
-                cost[out] = (unsigned)min(ns * (long long)((kWave * ipl) >> glog), 0xffffffffll);
+                cost[out] = (unsigned)min(ns * (long long)(kWave * ipl / (kWave / lanes)), 0xffffffffll);
             ++out;
         }
     };

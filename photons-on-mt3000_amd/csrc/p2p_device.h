@@ -8,8 +8,27 @@
 namespace pp2p {
 
 constexpr int kWave = 64;            // CDNA wavefront width
-constexpr int kWavesPerGroup = 4;    // force kernel workgroup = 256 lanes
+constexpr int kWavesPerGroup = 8;    // force kernel workgroup = 512 lanes
 constexpr int kTileCap = 128;        // j-tile capacity per wave (particles)
+
+// LDS of one force-kernel workgroup: one 64 KiB array at LDS address 0.
+//   [0, 16K)       j-tiles, kTileCap float4 per wave
+//   [24K, 32K)     LOGTAB table, 16 segments per octave
+//   [32K, 48K)     source directories, 2 KiB per wave
+//   [48K, 64K)     LOGTAB table, 32 segments per octave
+// The table sits where the bits of s address it directly: with s in
+// [2^-31, 1] (biased exponent e in [96, 127]) the byte address of the entry
+// of s is (bits(s) & ~0 << seg_shift) >> (seg_shift - 4) = e * 16 * segs + 16 *
+// (top mantissa bits), i.e. 96 * 16 * segs bytes plus the cyclic entry index
+// ((e mod 32) * segs + mantissa bits) times 16.  Two workgroups fit a CU's
+// 160 KiB, 16 waves per CU.
+constexpr int kLdsBytes = 65536;
+constexpr int kLdsTileOff = 0;
+constexpr int kLdsDirOff = 32768;
+constexpr int kLdsDirBytes = 2048;   // sizeof(SrcDir)
+__host__ __device__ constexpr int lds_table_off(int seg_shift) { return 96 << (27 - seg_shift); }
+static_assert(kLdsTileOff + kWavesPerGroup * kTileCap * 16 <= 24576, "tiles overlap the 16-segment table");
+static_assert(kLdsDirOff + kWavesPerGroup * kLdsDirBytes <= 49152, "directories overlap the 32-segment table");
 constexpr int kLogOctaves = 32;      // LOGTAB: cyclic table spans 32 binary octaves of s
 constexpr int kLogSegsMax = 32;      // LOGTAB: segments per octave (16 or 32)
 constexpr int kLogTabMax = kLogOctaves * kLogSegsMax;
@@ -32,7 +51,7 @@ struct alignas(16) Item {
     int row;      // CSR row
     int s_beg;    // source range [s_beg, s_end) into src[]
     int s_end;
-    int pad0;     // log2 of the number of phases G
+    int pad0;     // lanes per phase L (1..64); G = 64 / L phases
     int pad1;     // i-particles per lane (IPL), 1..4
 };
 
@@ -43,7 +62,8 @@ struct LawConst {
     int   seg_bits;   // LOGTAB: 5 + log2(segments per octave)
     float k2;         // POLY: -ucut^2 log2(e), so exp2(k2 s) = exp(-u^2)
     float poly[kPolyDeg + 1];   // POLY: h(r) coefficients, highest degree first
-    float pad[3];
+    int   tab_shift;  // LOGTAB: seg_shift - 4 (entry byte address = segment start bits >> tab_shift)
+    float pad[2];
 };
 
 struct ForceArgs {

@@ -1,17 +1,21 @@
 // gfx950 kernels of the short-range P2P path.
 //
-// Force kernel: every wave owns one work item at a time (up to 64 i-particles
-// of a target leaf against a range of its source leaves).  i-particles live
-// in registers; each source leaf is staged through a wave-private LDS tile in
-// chunks of 64 particles, converted on the way in from the source leaf's frame
-// to the target leaf's frame with an FP64 offset (and a minimum-image wrap
-// when the box is periodic).  Targets smaller than 64 particles split the
-// j-loop of each chunk over G = 64 / n_i lane groups ("phases"), which are
-// summed at the end.  Pair arithmetic is FP32 with the hardware v_rsq_f32;
-// the per-chunk partial sums fold into FP64.
+// Force kernel: persistent waves pop work items (i-particles of one target
+// leaf against a range of its CSR source leaves) from a cost-sorted queue.
+// i-particles live in registers, up to four per lane; the source particles of
+// the item run through a wave-private LDS tile in windows of 64 that span
+// consecutive source leaves, each particle moved from its leaf's frame into
+// the target leaf's frame with an FP64 offset (minimum image when the box is
+// periodic), culled against the item's bounding box and compacted.  Targets
+// smaller than 64 lane slots split every batch of the tile over G lane groups
+// ("phases") that meet by a butterfly at the end.  Pair arithmetic is FP32:
+// the LOGTAB law needs 13 FP32 and 3 integer VALU operations per pair and no
+// transcendental; FP32 partial sums fold into FP64 once per batch.
 //
 // Reference semantics (what is computed, not how): P2P_core / P2P_core_1 /
 // P2P_kernel in src/gpu_kernel.impl.h:50-157, :260-362, :364-437.
+#include <type_traits>
+
 #include "p2p_device.h"
 
 namespace pp2p {
@@ -32,6 +36,7 @@ struct SrcDir {
     int base[kWave];      // pos index minus window position of each source's first particle
     double ox[kWave], oy[kWave], oz[kWave];   // FP64 frame offsets (target frame - source frame)
 };
+static_assert(sizeof(SrcDir) == kLdsDirBytes, "source directory size");
 
 // inclusive prefix sum over the wave (lane order)
 __device__ __forceinline__ int wave_incl_scan(int v, int lane)
@@ -72,22 +77,34 @@ __device__ __forceinline__ float wave_max(float v)
 
 // s = r_eff^2 / rc^2 with the softening rule, clamped to s <= 1 (the cutoff).
 // Plummer folds eps^2 into the first product of r^2 (one FMA, no extra add).
+// The clamps compare bit patterns as unsigned integers, which orders
+// non-negative floats like their values (integer VALU ops issue at twice the
+// FP32 rate); any NaN pattern is above the pattern of 1.0, so the result lies
+// in [smin, 1] for every input.
 template <int SOFT>
 __device__ __forceinline__ float soft_s(float dx, float dy, float dz, const LawConst &lc)
 {
-    if (SOFT == SOFT_REF)
-        return __builtin_amdgcn_fmed3f(fmaf(dz, dz, fmaf(dy, dy, dx * dx)), lc.smin, 1.0f);   // max(r, eps), clamp
-    return fminf(fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, lc.smin))), 1.0f);
+    constexpr unsigned kOne = 0x3f800000u;
+    if (SOFT == SOFT_REF) {                                 // max(r, eps), clamp
+        const unsigned r2 = __float_as_uint(fmaf(dz, dz, fmaf(dy, dy, dx * dx)));
+        return __uint_as_float(min(max(r2, __float_as_uint(lc.smin)), kOne));
+    }
+    return __uint_as_float(min(__float_as_uint(fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, lc.smin)))), kOne));
 }
 
 // One pair, LOGTAB law.  F(s) = g(u(s)) / s^{3/2} is a cubic in (s - s0) on
-// each of `segs` segments per binary octave of s; the segment is named by the
-// exponent and top mantissa bits of s itself (cyclic over 32 octaves), so no
-// square root or reciprocal square root is evaluated.  The octave [1, 2) holds
-// zero coefficients: every pair at or beyond the cutoff contributes nothing.
+// each of `segs` segments per binary octave of s.  The segment start s0 is s
+// with its low mantissa bits cleared, and the same bits shifted right are the
+// byte address of the segment's coefficients in LDS (the table lies at the
+// LDS offset where s in [2^-31, 1] lands, p2p_device.h), so a pair costs one
+// AND, one SUB and one shift besides the FMAs: no square root, reciprocal
+// square root or index arithmetic.  The octave [1, 2) holds zero
+// coefficients: every pair at or beyond the cutoff contributes nothing.
+// s always lies in [smin, 1] (smin >= 2^-31 is checked on the host; the
+// clamps map a NaN to smin or 1), so the address stays inside the table.
 template <int SOFT>
 __device__ __forceinline__ void pair_logtab(float xi, float yi, float zi, float4 pj,
-                                            const float4 *__restrict__ tab,
+                                            const char *__restrict__ lds, unsigned seg_mask, int tab_shift,
                                             const LawConst &lc,
                                             float &ax, float &ay, float &az)
 {
@@ -95,11 +112,9 @@ __device__ __forceinline__ void pair_logtab(float xi, float yi, float zi, float4
     const float dy = pj.y - yi;
     const float dz = pj.z - zi;
     const float s = soft_s<SOFT>(dx, dy, dz, lc);
-    const unsigned bits = __float_as_uint(s);
-    const unsigned seg = __builtin_amdgcn_ubfe(bits, lc.seg_shift, lc.seg_bits);
-    const float s0 = __uint_as_float(bits & (0xffffffffu << lc.seg_shift));
-    const float t = s - s0;
-    const float4 c = tab[seg];
+    const unsigned b0 = __float_as_uint(s) & seg_mask;
+    const float t = s - __uint_as_float(b0);
+    const float4 c = *reinterpret_cast<const float4 *>(lds + (b0 >> tab_shift));
     const float f = fmaf(fmaf(fmaf(c.w, t, c.z), t, c.y), t, c.x);
     ax = fmaf(dx, f, ax);
     ay = fmaf(dy, f, ay);
@@ -140,24 +155,28 @@ __device__ __forceinline__ float4 null_particle() { return make_float4(1e18f, 1e
 #endif
 
 // One work item with IPL i-particles per lane.  Lane l belongs to phase
-// p = l / L (L = 64 / G lanes per phase, G = 2^glog phases) and holds the
-// i-slots q, q + L, ..., q + (IPL-1) L of the item, q = l % L; slots past
-// n_i hold a far-away dummy.  Source particles are staged into the tile in
-// batches of 64; phase p takes tile entries p, p + G, p + 2G, ... (lanes of
-// one phase read one address, neighbouring phases neighbouring addresses), so
-// every j read feeds IPL independent pairs.  The G partial sums of an
-// i-particle meet by a butterfly over lanes.
+// p = l / L (L lanes per phase, G = 64 / L phases; lanes from G * L on are
+// idle) and holds the i-slots q, q + L, ..., q + (IPL-1) L of the item,
+// q = l % L; slots past n_i hold a far-away dummy.  Source particles are
+// staged into the tile and consumed in batches of G * floor(64 / G); phase p
+// takes tile entries p, p + G, p + 2G, ... (lanes of one phase read one
+// address, neighbouring phases neighbouring addresses), so every j read
+// feeds IPL independent pairs.  The G partial sums of an i-particle meet by
+// a butterfly over lanes (G a power of two) or a chain of lane reads.
 template <int IPL, int LAW, int SOFT, bool FOLD64, bool INDIRECT, bool COUNT>
 __device__ __forceinline__ void process_item(const ForceArgs &a, const LawConst &lc, const Item &it,
-                                             const float4 *__restrict__ s_tab, float4 *tile, SrcDir &dir,
+                                             const char *__restrict__ lds, float4 *tile, SrcDir &dir,
                                              int lane, unsigned long long &n_in_total)
 {
     const int ni = it.ni;
-    const int glog = it.pad0;
-    const int G = 1 << glog;
-    const int L = kWave >> glog;
-    const int phase = lane >> (6 - glog);
-    const int q = lane & (L - 1);
+    const int L = it.pad0;                          // lanes per phase
+    const int G = kWave / L;                        // phases
+    const int phase = lane / L;
+    const int q = lane - phase * L;
+    const bool live = phase < G;
+    const int rphase = live ? phase : 0;            // idle lanes shadow phase 0 with dummy i-slots
+    const int glog = (G & (G - 1)) == 0 ? 31 - __builtin_clz(G) : -1;   // G = 2^glog, or -1
+    const int nb = G * (kWave / G);                 // batch: the largest multiple of G <= 64
     const unsigned long long below = (1ull << lane) - 1ull;
 
     float xi[IPL], yi[IPL], zi[IPL];
@@ -167,7 +186,7 @@ __device__ __forceinline__ void process_item(const ForceArgs &a, const LawConst 
 #pragma unroll
     for (int k = 0; k < IPL; ++k) {
         const int il = q + k * L;
-        valid[k] = il < ni;
+        valid[k] = live && il < ni;
         ip[k] = 0;
         xi[k] = yi[k] = zi[k] = 1e18f;
         if (valid[k]) {
@@ -199,23 +218,51 @@ __device__ __forceinline__ void process_item(const ForceArgs &a, const LawConst 
     unsigned n_in = 0;
     int cnt = 0;                                   // j-particles waiting in the tile
 
-    auto run_batch = [&](int n) {                  // n: multiple of G
-        const int ch = n >> glog;
-        const float4 *tp = tile + phase;
+    const unsigned seg_mask = 0xffffffffu << lc.seg_shift;
+    const int tab_shift = lc.tab_shift;
+    auto pairs_of = [&](const float4 &pj) {
+#pragma unroll
+        for (int k = 0; k < IPL; ++k) {
+            if (COUNT) {
+                const float dx = pj.x - xi[k], dy = pj.y - yi[k], dz = pj.z - zi[k];
+                n_in += valid[k] && soft_s<SOFT>(dx, dy, dz, lc) < 1.0f;
+            } else if (LAW == LAW_LOGTAB) {
+                pair_logtab<SOFT>(xi[k], yi[k], zi[k], pj, lds, seg_mask, tab_shift, lc, fx[k], fy[k], fz[k]);
+            } else {
+                pair_poly<SOFT>(xi[k], yi[k], zi[k], pj, lc, fx[k], fy[k], fz[k]);
+            }
+        }
+    };
+    // one batch of n (a multiple of G) tile entries.  With G = 2^GL a template
+    // constant the tile addresses of an unrolled step are immediate offsets of
+    // one pointer; other G step the pointer by G.
+    auto batch_g = [&](auto glc, int n) {
+        constexpr int GL = decltype(glc)::value;
+        const int ch = n >> GL;
+        const float4 *tp = tile + rphase;
+#pragma unroll 2
+        for (int u = 0; u < ch; ++u)
+            pairs_of(tp[u << GL]);
+    };
+    auto batch_any = [&](int n) {
+        const int ch = n / G;
+        const float4 *tp = tile + rphase;
 #pragma unroll 2
         for (int u = 0; u < ch; ++u) {
-            const float4 pj = tp[u << glog];
-#pragma unroll
-            for (int k = 0; k < IPL; ++k) {
-                if (COUNT) {
-                    const float dx = pj.x - xi[k], dy = pj.y - yi[k], dz = pj.z - zi[k];
-                    n_in += valid[k] && soft_s<SOFT>(dx, dy, dz, lc) < 1.0f;
-                } else if (LAW == LAW_LOGTAB) {
-                    pair_logtab<SOFT>(xi[k], yi[k], zi[k], pj, s_tab, lc, fx[k], fy[k], fz[k]);
-                } else {
-                    pair_poly<SOFT>(xi[k], yi[k], zi[k], pj, lc, fx[k], fy[k], fz[k]);
-                }
-            }
+            pairs_of(*tp);
+            tp += G;
+        }
+    };
+    auto run_batch = [&](int n) {
+        switch (glog) {
+        case 0: batch_g(std::integral_constant<int, 0>{}, n); break;
+        case 1: batch_g(std::integral_constant<int, 1>{}, n); break;
+        case 2: batch_g(std::integral_constant<int, 2>{}, n); break;
+        case 3: batch_g(std::integral_constant<int, 3>{}, n); break;
+        case 4: batch_g(std::integral_constant<int, 4>{}, n); break;
+        case 5: batch_g(std::integral_constant<int, 5>{}, n); break;
+        case 6: batch_g(std::integral_constant<int, 6>{}, n); break;
+        default: batch_any(n); break;
         }
         if (FOLD64) {
 #pragma unroll
@@ -299,13 +346,15 @@ __device__ __forceinline__ void process_item(const ForceArgs &a, const LawConst 
                 tile[cnt + __popcll(m & below)] = make_float4(w.x, w.y, w.z, 0.f);
             cnt += __popcll(m);
             wave_lds_sync();
-            if (cnt >= kWave) {
-                run_batch(kWave);
+            // cnt < nb before the window was added, so cnt < nb + 64 <= kTileCap
+            // and every remainder below is < 64 (one lane per entry)
+            while (cnt >= nb) {
+                run_batch(nb);
                 wave_lds_sync();
-                const int rem = cnt - kWave;           // < 64: no overlap
+                const int rem = cnt - nb;
                 float4 v;
                 if (lane < rem)
-                    v = tile[kWave + lane];
+                    v = tile[nb + lane];
                 wave_lds_sync();
                 if (lane < rem)
                     tile[lane] = v;
@@ -319,7 +368,7 @@ __device__ __forceinline__ void process_item(const ForceArgs &a, const LawConst 
         wave_lds_sync();                               // directory reused by the next batch
     }
     if (cnt > 0) {
-        const int n = ((cnt + G - 1) >> glog) << glog;
+        const int n = (cnt + G - 1) / G * G;
         if (lane < n - cnt)
             tile[cnt + lane] = null_particle();
         wave_lds_sync();
@@ -335,11 +384,22 @@ __device__ __forceinline__ void process_item(const ForceArgs &a, const LawConst 
         if (!FOLD64) {
             sx[k] = (double)fx[k]; sy[k] = (double)fy[k]; sz[k] = (double)fz[k];
         }
-        // butterfly over the phases: lanes q, q + L, q + 2L, ... hold slot k of one i
-        for (int s = L; s < kWave; s <<= 1) {
-            sx[k] += __shfl_xor(sx[k], s, kWave);
-            sy[k] += __shfl_xor(sy[k], s, kWave);
-            sz[k] += __shfl_xor(sz[k], s, kWave);
+        // lanes q, q + L, ..., q + (G-1) L hold slot k of one i: sum into lane q
+        // (a butterfly needs L and G both powers of two, i.e. G * L = 64)
+        if (G * L == kWave) {
+            for (int s = L; s < kWave; s <<= 1) {
+                sx[k] += __shfl_xor(sx[k], s, kWave);
+                sy[k] += __shfl_xor(sy[k], s, kWave);
+                sz[k] += __shfl_xor(sz[k], s, kWave);
+            }
+        } else {
+            const double x0 = sx[k], y0 = sy[k], z0 = sz[k];
+            for (int m = 1; m < G; ++m) {
+                const int src = q + m * L;                  // < G * L <= 64 for phase-0 lanes
+                sx[k] += __shfl(x0, src, kWave);
+                sy[k] += __shfl(y0, src, kWave);
+                sz[k] += __shfl(z0, src, kWave);
+            }
         }
         if (phase == 0 && valid[k]) {
             unsafeAtomicAdd(a.acc + ip[k], sx[k] * a.out_scale);
@@ -356,20 +416,21 @@ __global__ __launch_bounds__(kWave * kWavesPerGroup)
 __attribute__((amdgpu_waves_per_eu(PP2P_WAVES_PER_EU)))
 void p2p_force(ForceArgs a, LawConst lc)
 {
-    extern __shared__ float4 s_tab[];                       // LOGTAB table
-    __shared__ float4 s_tile[kWavesPerGroup][kTileCap];     // j-tiles, one per wave
-    __shared__ SrcDir s_dir[kWavesPerGroup];                // source directories, one per wave
+    __shared__ float4 s_lds[kLdsBytes / 16];                // the workgroup's whole LDS (layout: p2p_device.h)
+    char *lds = reinterpret_cast<char *>(s_lds);
 
     const int wv = threadIdx.x / kWave;
     const int lane = threadIdx.x % kWave;
 
     if (LAW == LAW_LOGTAB) {
+        float4 *tab = reinterpret_cast<float4 *>(lds + lds_table_off(lc.seg_shift));
         for (int k = threadIdx.x; k < a.table_len; k += blockDim.x)
-            s_tab[k] = a.table[k];
+            tab[k] = a.table[k];
         __syncthreads();
     }
 
-    float4 *tile = s_tile[wv];
+    float4 *tile = reinterpret_cast<float4 *>(lds + kLdsTileOff) + wv * kTileCap;
+    SrcDir &dir = *reinterpret_cast<SrcDir *>(lds + kLdsDirOff + wv * kLdsDirBytes);
     int item_id = QUEUE ? 0 : a.item_base + blockIdx.x * kWavesPerGroup + wv;
     unsigned long long n_in = 0;
 
@@ -384,10 +445,10 @@ void p2p_force(ForceArgs a, LawConst lc)
             break;
         const Item it = a.items[item_id];
         switch (it.pad1) {
-        case 1: process_item<1, LAW, SOFT, FOLD64, INDIRECT, COUNT>(a, lc, it, s_tab, tile, s_dir[wv], lane, n_in); break;
-        case 2: process_item<2, LAW, SOFT, FOLD64, INDIRECT, COUNT>(a, lc, it, s_tab, tile, s_dir[wv], lane, n_in); break;
-        case 3: process_item<3, LAW, SOFT, FOLD64, INDIRECT, COUNT>(a, lc, it, s_tab, tile, s_dir[wv], lane, n_in); break;
-        default: process_item<4, LAW, SOFT, FOLD64, INDIRECT, COUNT>(a, lc, it, s_tab, tile, s_dir[wv], lane, n_in); break;
+        case 1: process_item<1, LAW, SOFT, FOLD64, INDIRECT, COUNT>(a, lc, it, lds, tile, dir, lane, n_in); break;
+        case 2: process_item<2, LAW, SOFT, FOLD64, INDIRECT, COUNT>(a, lc, it, lds, tile, dir, lane, n_in); break;
+        case 3: process_item<3, LAW, SOFT, FOLD64, INDIRECT, COUNT>(a, lc, it, lds, tile, dir, lane, n_in); break;
+        default: process_item<4, LAW, SOFT, FOLD64, INDIRECT, COUNT>(a, lc, it, lds, tile, dir, lane, n_in); break;
         }
         if (!QUEUE)
             break;
@@ -411,9 +472,8 @@ namespace pp2p {
 template <int LAW, int SOFT, bool FOLD64, bool INDIRECT, bool QUEUE>
 static hipError_t launch_one(const ForceArgs &a, const LawConst &lc, int grid, hipStream_t st)
 {
-    const size_t lds = LAW == LAW_LOGTAB ? sizeof(float4) * (size_t)a.table_len : 0;
     hipLaunchKernelGGL((p2p_force<LAW, SOFT, FOLD64, INDIRECT, QUEUE>), dim3(grid),
-                       dim3(kWave * kWavesPerGroup), lds, st, a, lc);
+                       dim3(kWave * kWavesPerGroup), 0, st, a, lc);
     return hipGetLastError();
 }
 
@@ -449,17 +509,17 @@ hipError_t launch_force(const ForceArgs &a, const LawConst &lc, const ForceVaria
 
 hipError_t force_occupancy(const ForceVariant &v, int table_len, int *groups_per_cu)
 {
-    const size_t lds = v.law == LAW_LOGTAB ? sizeof(float4) * (size_t)table_len : 0;
+    (void)table_len;                                        // the table lives in the static LDS array
     if (v.law == LAW_LOGTAB)
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(groups_per_cu,
-                (const void *)p2p_force<LAW_LOGTAB, SOFT_REF, true, false, true>, kWave * kWavesPerGroup, lds);
+                (const void *)p2p_force<LAW_LOGTAB, SOFT_REF, true, false, true>, kWave * kWavesPerGroup, 0);
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(groups_per_cu,
-            (const void *)p2p_force<LAW_POLY, SOFT_REF, true, false, true>, kWave * kWavesPerGroup, lds);
+            (const void *)p2p_force<LAW_POLY, SOFT_REF, true, false, true>, kWave * kWavesPerGroup, 0);
 }
 
 hipError_t launch_count(const ForceArgs &a, const LawConst &lc, const ForceVariant &v, int grid, hipStream_t st)
 {
-    const size_t lds = sizeof(float4) * (size_t)a.table_len;
+    const size_t lds = 0;
     if (v.soft == SOFT_REF) {
         if (v.indirect)
             hipLaunchKernelGGL((p2p_force<LAW_LOGTAB, SOFT_REF, true, true, true, true>), dim3(grid),

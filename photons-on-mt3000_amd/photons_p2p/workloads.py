@@ -218,7 +218,7 @@ def build_lists(posi: np.ndarray, tag: np.ndarray, int2pos: float, maxleaf: int,
 
 def tree_workload(name: str, pos: np.ndarray, box: float, mesh: int, maxleaf: int, open_angle: float,
                   eps: float, softening: int, periodic_ghosts: bool = True, nthreads: int = 0,
-                  meta: dict | None = None) -> Workload:
+                  meta: dict | None = None, progress=None) -> Workload:
     """Local particles in [0, box) -> reference integer grid, ghost copies
     within cutoffRadius, tree + walk lists.  The returned positions are the
     grid positions (posi * INT2POS, FP64) in tree order, ghosts included;
@@ -238,6 +238,8 @@ def tree_workload(name: str, pos: np.ndarray, box: float, mesh: int, maxleaf: in
     tag = np.concatenate([np.ones(posi.shape[0], np.int32), np.zeros(gpos.shape[0], np.int32)])
     local_index = np.concatenate([np.arange(posi.shape[0]), gsrc])
     bdl, bdr = combine_bounds(allp, int2pos, box)
+    if progress:
+        progress(f"{allp.shape[0]} particles ({gpos.shape[0]} ghost copies) on the grid; building tree + lists")
     lists = build_lists(allp, tag, int2pos, maxleaf, np.float32(open_angle), np.float32(cutoff_radius),
                         bdl, bdr, nthreads)
     perm = lists["perm"]
@@ -373,14 +375,16 @@ def config_plain(n1: int = 64, box: float = 100.0, seed: int = 20261202, softeni
 
 def config_clustered(n1: int = 256, box: float = 400.0, seed: int = 20261203, softening: int = 1,
                      maxleaf: int = 64, open_angle: float = 0.5, n_spheres: int = 2000,
-                     nthreads: int = 0) -> Workload:
+                     nthreads: int = 0, progress=None) -> Workload:
     """Plain P2P at scale: half Zel'dovich grid, half in Plummer spheres;
     leaves <= 64, mesh n1^3, Plummer eps = spacing / 40."""
     pos = clustered_mix(n1, box, seed, n_spheres)
+    if progress:
+        progress(f"{pos.shape[0]} particles generated")
     d = box / n1
     eps = d / 40.0 if softening == 1 else 0.03 * d
     return tree_workload(f"clustered{n1}", pos, box, n1, maxleaf, open_angle, eps, softening, nthreads=nthreads,
-                         meta={"seed": seed, "n1": n1, "n_spheres": n_spheres})
+                         meta={"seed": seed, "n1": n1, "n_spheres": n_spheres}, progress=progress)
 
 
 def config_precision(n1: int = 128, box: float = 100.0, seed: int = 20261204, n_halos: int = 500,
