@@ -13,7 +13,7 @@ struct RowArgs {
     const int *act_off;   // per leaf offset into ilist
     int nrows;
     float stage_ops;      // > 0: most i-particles per item (default 64 * ipl_max)
-    int ipl_max;          // most i-particles per lane, 1..4
+    int ipl_max;          // most i-particles per lane, 1..3
 };
 
 hipError_t launch_leaf_frame(const double *x3, const int2 *rng, int nleaf, double inv_rc, bool frame_abs,

@@ -166,7 +166,7 @@ struct RowView {
     const int2 *rng;
     const int *act_cnt, *act_off;   // null: all particles of the target
     float stage_ops;                 // > 0: most i-particles per item
-    int ipl_max;                     // most i-particles per lane, 1..4
+    int ipl_max;                     // most i-particles per lane, 1..3
 };
 
 __device__ __forceinline__ void row_i(const RowView &v, int tgt, int &ibeg, int &ni)

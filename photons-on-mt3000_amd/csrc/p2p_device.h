@@ -33,7 +33,7 @@ constexpr int kLogOctaves = 32;      // LOGTAB: cyclic table spans 32 binary oct
 constexpr int kLogSegsMax = 32;      // LOGTAB: segments per octave (16 or 32)
 constexpr int kLogTabMax = kLogOctaves * kLogSegsMax;
 constexpr int kPolyDeg = 12;         // POLY: degree of h(r)
-constexpr int kIplMax = 4;           // i-particles per lane, at most
+constexpr int kIplMax = 3;           // i-particles per lane, at most
 constexpr int kItemMaxI = kWave * kIplMax;   // i-particles per work item
 
 // Units inside the kernels: lengths are divided by the cutoff radius rc

@@ -32,11 +32,9 @@ __device__ __forceinline__ void wave_lds_sync()
 
 // Per-wave LDS directory of up to 64 source leaves (see process_item).
 struct SrcDir {
-    int pre[kWave];       // inclusive prefix of the source sizes
-    int base[kWave];      // pos index minus window position of each source's first particle
     double ox[kWave], oy[kWave], oz[kWave];   // FP64 frame offsets (target frame - source frame)
 };
-static_assert(sizeof(SrcDir) == kLdsDirBytes, "source directory size");
+static_assert(sizeof(SrcDir) <= kLdsDirBytes, "source directory size");
 
 // inclusive prefix sum over the wave (lane order)
 __device__ __forceinline__ int wave_incl_scan(int v, int lane)
@@ -92,7 +90,7 @@ __device__ __forceinline__ float soft_s(float dx, float dy, float dz, const LawC
     return __uint_as_float(min(__float_as_uint(fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, lc.smin)))), kOne));
 }
 
-// One pair, LOGTAB law.  F(s) = g(u(s)) / s^{3/2} is a cubic in (s - s0) on
+// LOGTAB law.  F(s) = g(u(s)) / s^{3/2} is a cubic in (s - s0) on
 // each of `segs` segments per binary octave of s.  The segment start s0 is s
 // with its low mantissa bits cleared, and the same bits shifted right are the
 // byte address of the segment's coefficients in LDS (the table lies at the
@@ -102,23 +100,17 @@ __device__ __forceinline__ float soft_s(float dx, float dy, float dz, const LawC
 // coefficients: every pair at or beyond the cutoff contributes nothing.
 // s always lies in [smin, 1] (smin >= 2^-31 is checked on the host; the
 // clamps map a NaN to smin or 1), so the address stays inside the table.
-template <int SOFT>
-__device__ __forceinline__ void pair_logtab(float xi, float yi, float zi, float4 pj,
-                                            const char *__restrict__ lds, unsigned seg_mask, int tab_shift,
-                                            const LawConst &lc,
-                                            float &ax, float &ay, float &az)
+__device__ __forceinline__ float4 logtab_segment(float s, const char *__restrict__ lds, unsigned seg_mask,
+                                                 int tab_shift, float &t)
 {
-    const float dx = pj.x - xi;
-    const float dy = pj.y - yi;
-    const float dz = pj.z - zi;
-    const float s = soft_s<SOFT>(dx, dy, dz, lc);
     const unsigned b0 = __float_as_uint(s) & seg_mask;
-    const float t = s - __uint_as_float(b0);
-    const float4 c = *reinterpret_cast<const float4 *>(lds + (b0 >> tab_shift));
-    const float f = fmaf(fmaf(fmaf(c.w, t, c.z), t, c.y), t, c.x);
-    ax = fmaf(dx, f, ax);
-    ay = fmaf(dy, f, ay);
-    az = fmaf(dz, f, az);
+    t = s - __uint_as_float(b0);
+    return *reinterpret_cast<const float4 *>(lds + (b0 >> tab_shift));
+}
+
+__device__ __forceinline__ float logtab_eval(float4 c, float t)
+{
+    return fmaf(fmaf(fmaf(c.w, t, c.z), t, c.y), t, c.x);
 }
 
 // One pair, POLY law: F = exp(-u^2) h(r) / r^3 with h(r) = erfcx(u) + (2/sqrt(pi)) u
@@ -180,31 +172,34 @@ __device__ __forceinline__ void process_item(const ForceArgs &a, const LawConst 
     const unsigned long long below = (1ull << lane) - 1ull;
 
     float xi[IPL], yi[IPL], zi[IPL];
-    int ip[IPL];
     bool valid[IPL];
     float bxl = 3.0e38f, bxh = -3.0e38f, byl = 3.0e38f, byh = -3.0e38f, bzl = 3.0e38f, bzh = -3.0e38f;
+    auto i_index = [&](int k) {                     // particle index of slot k (valid slots)
+        const int il = q + k * L;
+        return INDIRECT ? a.ilist[it.i_beg + il] : it.i_beg + il;
+    };
 #pragma unroll
     for (int k = 0; k < IPL; ++k) {
         const int il = q + k * L;
         valid[k] = live && il < ni;
-        ip[k] = 0;
         xi[k] = yi[k] = zi[k] = 1e18f;
         if (valid[k]) {
-            ip[k] = INDIRECT ? a.ilist[it.i_beg + il] : it.i_beg + il;
-            const float4 p = a.pos[ip[k]];
+            const float4 p = a.pos[i_index(k)];
             xi[k] = p.x; yi[k] = p.y; zi[k] = p.z;
             bxl = fminf(bxl, p.x); bxh = fmaxf(bxh, p.x);
             byl = fminf(byl, p.y); byh = fmaxf(byh, p.y);
             bzl = fminf(bzl, p.z); bzh = fmaxf(bzh, p.z);
         }
     }
-    // bounding box of the item's i-particles (lane 0 always holds a valid slot)
+    // bounding box of the item's i-particles (lane 0 always holds a valid
+    // slot), kept in scalar registers
+    auto uniform = [](float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); };
     bxl = wave_min(bxl); bxh = wave_max(bxh);
     byl = wave_min(byl); byh = wave_max(byh);
     bzl = wave_min(bzl); bzh = wave_max(bzh);
-    const float cx = 0.5f * (bxl + bxh), hx = 0.5f * (bxh - bxl);
-    const float cy = 0.5f * (byl + byh), hy = 0.5f * (byh - byl);
-    const float cz = 0.5f * (bzl + bzh), hz = 0.5f * (bzh - bzl);
+    const float cx = uniform(0.5f * (bxl + bxh)), hx = uniform(0.5f * (bxh - bxl));
+    const float cy = uniform(0.5f * (byl + byh)), hy = uniform(0.5f * (byh - byl));
+    const float cz = uniform(0.5f * (bzl + bzh)), hz = uniform(0.5f * (bzh - bzl));
     const float cull_lim = a.cull ? 1.0009765625f : 3.0e38f;   // s > 1 + 2^-10: no pair can reach
 
     const double4 rt = a.ref[it.tgt];
@@ -220,14 +215,31 @@ __device__ __forceinline__ void process_item(const ForceArgs &a, const LawConst 
 
     const unsigned seg_mask = 0xffffffffu << lc.seg_shift;
     const int tab_shift = lc.tab_shift;
+    // the IPL pairs of one j: all table addresses first, then the gathers,
+    // then the polynomials, so that the gathers of a step are in flight together
     auto pairs_of = [&](const float4 &pj) {
+        if (!COUNT && LAW == LAW_LOGTAB) {
+            float dx[IPL], dy[IPL], dz[IPL], t[IPL];
+            float4 c[IPL];
+#pragma unroll
+            for (int k = 0; k < IPL; ++k) {
+                dx[k] = pj.x - xi[k]; dy[k] = pj.y - yi[k]; dz[k] = pj.z - zi[k];
+                c[k] = logtab_segment(soft_s<SOFT>(dx[k], dy[k], dz[k], lc), lds, seg_mask, tab_shift, t[k]);
+            }
+#pragma unroll
+            for (int k = 0; k < IPL; ++k) {
+                const float f = logtab_eval(c[k], t[k]);
+                fx[k] = fmaf(dx[k], f, fx[k]);
+                fy[k] = fmaf(dy[k], f, fy[k]);
+                fz[k] = fmaf(dz[k], f, fz[k]);
+            }
+            return;
+        }
 #pragma unroll
         for (int k = 0; k < IPL; ++k) {
             if (COUNT) {
                 const float dx = pj.x - xi[k], dy = pj.y - yi[k], dz = pj.z - zi[k];
                 n_in += valid[k] && soft_s<SOFT>(dx, dy, dz, lc) < 1.0f;
-            } else if (LAW == LAW_LOGTAB) {
-                pair_logtab<SOFT>(xi[k], yi[k], zi[k], pj, lds, seg_mask, tab_shift, lc, fx[k], fy[k], fz[k]);
             } else {
                 pair_poly<SOFT>(xi[k], yi[k], zi[k], pj, lc, fx[k], fy[k], fz[k]);
             }
@@ -273,11 +285,13 @@ __device__ __forceinline__ void process_item(const ForceArgs &a, const LawConst 
         }
     };
 
-    // Sources 64 at a time: lane s loads source s's range and FP64 frame
-    // offset into the wave's LDS directory with the inclusive prefix of the
-    // source sizes.  Particles are then staged in windows of 64 that run over
-    // the concatenated sources (a lane finds its source by binary search in
-    // the prefix), the next window prefetched while the current tile computes.
+    // Sources 64 at a time: lane s holds source s's first particle (relative
+    // to the concatenated positions) and the inclusive prefix of the source
+    // sizes in registers, its FP64 frame offset in the wave's LDS directory.
+    // Particles are staged in windows of 64 that run over the concatenated
+    // sources.  The sources that meet a window are found by a wave-uniform
+    // walk over the prefix (lane reads, no memory round trip), and the next
+    // window is prefetched while the current one is staged and computed.
     for (int sb = it.s_beg; sb < it.s_end; sb += kWave) {
         const int nsrc = min(kWave, it.s_end - sb);
         int m_ip = 0, m_np = 0;
@@ -299,35 +313,47 @@ __device__ __forceinline__ void process_item(const ForceArgs &a, const LawConst 
         }
         const int pre = wave_incl_scan(m_np, lane);          // sources 0..lane
         const int total = __builtin_amdgcn_readlane(pre, kWave - 1);
-        dir.pre[lane] = pre;
-        dir.base[lane] = m_ip - (pre - m_np);               // pos index of window position 0 of source lane
+        const int base = m_ip - (pre - m_np);               // pos index of concatenated position 0, source lane
         dir.ox[lane] = m_ox;
         dir.oy[lane] = m_oy;
         dir.oz[lane] = m_oz;
         wave_lds_sync();
 
-        // window w covers concatenated positions [w, w + 64)
-        auto locate = [&](int J) -> int {                   // smallest s with pre[s] > J
-            int lo = 0;
-#pragma unroll
-            for (int step = 32; step > 0; step >>= 1)
-                if (dir.pre[lo + step - 1] <= J)
-                    lo += step;
-            return lo;
+        // source and pos-index base of position w0 + lane; `cur` (uniform) is
+        // a source at or before the one holding w0 and moves forward
+        int cur = 0;
+        auto locate = [&](int w0, int &ks, int &bs) {
+            int sc = cur;
+            ks = sc;
+            bs = __builtin_amdgcn_readlane(base, sc);
+            while (sc + 1 < nsrc) {
+                const int st = __builtin_amdgcn_readlane(pre, sc);   // first position of source sc + 1
+                if (st >= w0 + kWave)
+                    break;
+                ++sc;
+                const int bn = __builtin_amdgcn_readlane(base, sc);
+                if (w0 + lane >= st) {
+                    ks = sc;
+                    bs = bn;
+                }
+            }
+            cur = sc;
         };
         int J = lane;
-        int ks = J < total ? locate(J) : 0;
+        int ks, bs;
+        locate(0, ks, bs);
         float4 qv = make_float4(0.f, 0.f, 0.f, 0.f);
         if (J < total)
-            qv = a.pos[dir.base[ks] + J];
+            qv = a.pos[bs + J];
         for (int w0 = 0; w0 < total; w0 += kWave) {
             // prefetch the next window
             const int J2 = J + kWave;
-            int ks2 = 0;
+            int ks2 = 0, bs2 = 0;
             float4 q2 = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (J2 < total) {
-                ks2 = locate(J2);
-                q2 = a.pos[dir.base[ks2] + J2];
+            if (w0 + kWave < total) {
+                locate(w0 + kWave, ks2, bs2);
+                if (J2 < total)
+                    q2 = a.pos[bs2 + J2];
             }
             // stage the current window: target frame, cull, compact
             const bool has = J < total;
@@ -402,15 +428,16 @@ __device__ __forceinline__ void process_item(const ForceArgs &a, const LawConst 
             }
         }
         if (phase == 0 && valid[k]) {
-            unsafeAtomicAdd(a.acc + ip[k], sx[k] * a.out_scale);
-            unsafeAtomicAdd(a.acc + a.npart + ip[k], sy[k] * a.out_scale);
-            unsafeAtomicAdd(a.acc + 2 * a.npart + ip[k], sz[k] * a.out_scale);
+            const int ipk = i_index(k);
+            unsafeAtomicAdd(a.acc + ipk, sx[k] * a.out_scale);
+            unsafeAtomicAdd(a.acc + a.npart + ipk, sy[k] * a.out_scale);
+            unsafeAtomicAdd(a.acc + 2 * a.npart + ipk, sz[k] * a.out_scale);
         }
     }
 }
 
 // Persistent (QUEUE) or one-wave-per-item kernel.  The item's IPL selects
-// one of four specialisations; the branch is uniform per wave.
+// one of three specialisations; the branch is uniform per wave.
 template <int LAW, int SOFT, bool FOLD64, bool INDIRECT, bool QUEUE, bool COUNT = false>
 __global__ __launch_bounds__(kWave * kWavesPerGroup)
 __attribute__((amdgpu_waves_per_eu(PP2P_WAVES_PER_EU)))
@@ -447,8 +474,7 @@ void p2p_force(ForceArgs a, LawConst lc)
         switch (it.pad1) {
         case 1: process_item<1, LAW, SOFT, FOLD64, INDIRECT, COUNT>(a, lc, it, lds, tile, dir, lane, n_in); break;
         case 2: process_item<2, LAW, SOFT, FOLD64, INDIRECT, COUNT>(a, lc, it, lds, tile, dir, lane, n_in); break;
-        case 3: process_item<3, LAW, SOFT, FOLD64, INDIRECT, COUNT>(a, lc, it, lds, tile, dir, lane, n_in); break;
-        default: process_item<4, LAW, SOFT, FOLD64, INDIRECT, COUNT>(a, lc, it, lds, tile, dir, lane, n_in); break;
+        default: process_item<3, LAW, SOFT, FOLD64, INDIRECT, COUNT>(a, lc, it, lds, tile, dir, lane, n_in); break;
         }
         if (!QUEUE)
             break;

@@ -2,7 +2,7 @@
 
 Times candidate tunings of one prepared list on the device and keeps the
 fastest.  The knobs are the ones pp2p_tuning exposes:
-  ipl_max      i-particles per lane an item may use (1..4); with it the item
+  ipl_max      i-particles per lane an item may use (1..3); with it the item
                builder picks G phases x IPL slots per item
   split_pairs  size of a queue item (pair-interactions) = the j-range a wave
                owns before it fetches the next item
@@ -23,7 +23,7 @@ from . import Context, EXEC_QUEUE, ForceLaw, LAW_POLY, LAW_TABLE
 TUNED = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned.json")
 
 GRID = {
-    "ipl_max": [1, 2, 3, 4],
+    "ipl_max": [1, 2, 3],
     "split_pairs": [0, 1 << 16, 1 << 18, 1 << 20],
     "waves_per_cu": [0, 12, 8],
     "law": [LAW_TABLE, LAW_POLY],
