@@ -346,19 +346,10 @@ __device__ __forceinline__ void process_item(const ForceArgs &a, const LawConst 
         if (J < total)
             qv = a.pos[bs + J];
         for (int w0 = 0; w0 < total; w0 += kWave) {
-            // prefetch the next window
-            const int J2 = J + kWave;
-            int ks2 = 0, bs2 = 0;
-            float4 q2 = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (w0 + kWave < total) {
-                locate(w0 + kWave, ks2, bs2);
-                if (J2 < total)
-                    q2 = a.pos[bs2 + J2];
-            }
             // stage the current window: target frame, cull, compact
             const bool has = J < total;
             float4 w = qv;
-            if (FOLD64 && has) {
+            if (FOLD64) {                              // every lane's ks names a source of the block
                 w.x = (float)((double)qv.x + dir.ox[ks]);
                 w.y = (float)((double)qv.y + dir.oy[ks]);
                 w.z = (float)((double)qv.z + dir.oz[ks]);
@@ -371,6 +362,14 @@ __device__ __forceinline__ void process_item(const ForceArgs &a, const LawConst 
             if (keep)
                 tile[cnt + __popcll(m & below)] = make_float4(w.x, w.y, w.z, 0.f);
             cnt += __popcll(m);
+            // prefetch the next window now, so that its load overlaps the batches
+            // below; lanes past the end load the block's last particle
+            const int J2 = J + kWave;
+            if (w0 + kWave < total) {
+                locate(w0 + kWave, ks, bs);
+                qv = a.pos[bs + min(J2, total - 1)];
+            }
+            J = J2;
             wave_lds_sync();
             // cnt < nb before the window was added, so cnt < nb + 64 <= kTileCap
             // and every remainder below is < 64 (one lane per entry)
@@ -387,9 +386,6 @@ __device__ __forceinline__ void process_item(const ForceArgs &a, const LawConst 
                 cnt = rem;
                 wave_lds_sync();
             }
-            qv = q2;
-            ks = ks2;
-            J = J2;
         }
         wave_lds_sync();                               // directory reused by the next batch
     }
