@@ -37,6 +37,11 @@ for _p in (os.path.join(ROOT, "photons-on-mt3000_amd"), os.path.join(ROOT, "orac
 PEAK_FP32_TFLOPS = 157.3          # MI355X FP32 vector peak
 FLOPS_IN, FLOPS_OUT = 48, 11      # per-pair operation counts inside / beyond the cutoff
 METRIC = "G pair-interactions/s + % FP32 vector peak at p99 force err <=1e-5, 1 MI355X"
+# HBM bytes per force launch (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, factors from a
+# 1 GiB copy kernel profiled in the same runs); see the named profile summary
+MEASURED_TRAFFIC = {
+    "clustered256": {"bytes": 6.593e10, "profile": "profiles/p2p_force_clustered256_traffic.md"},
+}
 
 
 def log(*a):
@@ -168,6 +173,7 @@ def main():
     ap.add_argument("--split", type=int, default=0, help="queue item size in pair-interactions (0: auto)")
     ap.add_argument("--segs", type=int, default=0, help="table segments per octave (16/32, 0: 32)")
     ap.add_argument("--no-cull", action="store_true")
+    ap.add_argument("--ipl-max", type=int, default=0, help="most i-particles per lane of an item (0: library default)")
     ap.add_argument("--graph", type=int, default=0,
                     help="capture this many steps in one hipGraph and time its replays (0: eager steps)")
     ap.add_argument("--autotune", action="store_true",
@@ -216,7 +222,7 @@ def main():
     ctx = pp.Context(0)
     ctx.set_law(pp.ForceLaw(eps=w.eps, rs=w.rs, mass=w.mass, softening=w.softening, law=law, box=w.box))
     ctx.set_tuning(waves_per_cu=args.waves_per_cu, split_pairs=args.split, segs_per_octave=args.segs,
-                   no_cull=int(args.no_cull), chunk_model=args.chunk_model)
+                   no_cull=int(args.no_cull), chunk_model=args.chunk_model, ipl_max=args.ipl_max)
     ctx.set_particles(pos_eval, w.leaf_ip, w.leaf_np)
     if act is not None:
         ctx.set_active(act, 1)
@@ -227,7 +233,7 @@ def main():
     sig = AT.signature(st["ntask"], w.npart, w.nleaf, st["npairs"])
     lawp = pp.ForceLaw(eps=w.eps, rs=w.rs, mass=w.mass, softening=w.softening, law=law, box=w.box)
     tuned = None
-    explicit = args.waves_per_cu or args.split or args.chunk_model or args.segs or args.no_cull
+    explicit = args.waves_per_cu or args.split or args.chunk_model or args.segs or args.no_cull or args.ipl_max
     if args.autotune:
         res = AT.tune(ctx, lawp, log=log)
         tuned = {"source": "autotune", "signature": sig, **res}
@@ -382,7 +388,9 @@ def main():
         "active_sweep": sweep,
         "copy_calibration": copy_cal,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None},
+                     "frac": achieved / PEAK_FP32_TFLOPS,
+                     "traffic": MEASURED_TRAFFIC.get(w.name, {}).get("bytes") if args.active <= 0 else None,
+                     "traffic_source": MEASURED_TRAFFIC.get(w.name, {}).get("profile") if args.active <= 0 else None},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
