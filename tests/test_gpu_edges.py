@@ -1,0 +1,135 @@
+"""Edge cases of the P2P path on the GPU, each against the FP64 oracle on the
+same lists: empty and ragged inputs, leaves from 1 particle up to several
+work items, self pairs, repeated leaf pairs, periodic wrap, and the host-side
+rejection of bad input (no kernel runs on it)."""
+import numpy as np
+import pytest
+
+import photons_p2p as pp
+import pyoracle
+
+pytestmark = pytest.mark.gpu
+
+EPS, RS, BOX = 0.05, 1.0, 0.0
+
+
+def _oracle(pos, ip, npl, tt, ts, softening=1, box=BOX):
+    prm = pyoracle.params(EPS, RS, softening=softening, box=box)
+    csr = pyoracle.build_csr(tt, ts, ip.size)
+    return pyoracle.p2p_f64(prm, pos, ip, npl, csr)
+
+
+def _run(ctx, pos, ip, npl, tt, ts, softening=1, box=BOX, exec_form=pp.EXEC_QUEUE):
+    print(f"case: N={pos.shape[0]} leaves={ip.size} tasks={tt.size} soft={softening} box={box} "
+          f"exec={exec_form}", flush=True)
+    ctx.set_tuning()
+    ctx.set_law(pp.ForceLaw(eps=EPS, rs=RS, softening=softening, box=box))
+    ctx.set_particles(pos, ip, npl)
+    ctx.set_tasks(tt, ts)
+    ctx.compute(exec_form)
+    return ctx.acc()
+
+
+def _leaves(sizes, seed, spread=2.0, width=1.5):
+    """Leaves of the given sizes, each a cube of side `width` at a random
+    centre in a box of side spread * cutoff radius (2 rs ucut = 5.625)."""
+    g = np.random.default_rng(seed)
+    rc = 2 * RS * 2.8125
+    centres = g.random((len(sizes), 3)) * spread * rc
+    pos = np.concatenate([c + (g.random((n, 3)) - 0.5) * width for c, n in zip(centres, sizes)])
+    npl = np.asarray(sizes, np.int32)
+    ip = np.concatenate([[0], np.cumsum(npl)[:-1]]).astype(np.int32)
+    return pos, ip, npl
+
+
+def _check(a, ref, p99=1e-5, mx=1e-4):
+    med, q, m = pyoracle.rel_err(a, ref)
+    print(f"rel err median {med:.2e} p99 {q:.2e} max {m:.2e}")
+    assert q <= p99 and m <= mx
+
+
+def test_empty_list_leaves_zero(gpu_ctx):
+    pos, ip, npl = _leaves([5, 7], seed=1)
+    a = _run(gpu_ctx, pos, ip, npl, np.zeros(0, np.int32), np.zeros(0, np.int32))
+    assert np.all(a == 0.0)
+
+
+@pytest.mark.parametrize("exec_form", [pp.EXEC_QUEUE, pp.EXEC_GRID, pp.EXEC_PER_TARGET])
+def test_ragged_leaf_sizes(gpu_ctx, exec_form):
+    """1 .. 700 particles per leaf: every lane shape, i-chunks beyond one item,
+    all pairs of leaves listed both ways."""
+    sizes = [1, 2, 3, 7, 20, 31, 33, 48, 63, 64, 65, 96, 127, 191, 192, 193, 257, 700]
+    pos, ip, npl = _leaves(sizes, seed=2)
+    n = len(sizes)
+    tt, ts = np.meshgrid(np.arange(n, dtype=np.int32), np.arange(n, dtype=np.int32), indexing="ij")
+    tt, ts = tt.ravel(), ts.ravel()
+    ref = _oracle(pos, ip, npl, tt, ts)
+    a = _run(gpu_ctx, pos, ip, npl, tt, ts, exec_form=exec_form)
+    _check(a, ref)
+
+
+def test_self_pairs_only(gpu_ctx):
+    sizes = [1, 9, 64, 150]
+    pos, ip, npl = _leaves(sizes, seed=3, width=0.8)
+    tt = np.arange(len(sizes), dtype=np.int32)
+    ref = _oracle(pos, ip, npl, tt, tt)
+    a = _run(gpu_ctx, pos, ip, npl, tt, tt)
+    assert np.all(a[0] == 0.0)                    # a lone particle feels nothing from itself
+    _check(a[1:], ref[1:])
+
+
+def test_repeated_leaf_pairs_count_twice(gpu_ctx):
+    """A leaf pair listed twice contributes twice, as the reference's loop over
+    its task list does."""
+    pos, ip, npl = _leaves([40, 25, 60], seed=4)
+    tt = np.array([0, 0, 1, 2, 0], np.int32)
+    ts = np.array([1, 1, 2, 0, 2], np.int32)
+    ref = _oracle(pos, ip, npl, tt, ts)
+    a = _run(gpu_ctx, pos, ip, npl, tt, ts)
+    _check(a, ref)
+    once = _oracle(pos, ip, npl, tt[1:], ts[1:])
+    assert not np.allclose(ref[ip[0]:ip[0] + npl[0]], once[ip[0]:ip[0] + npl[0]])
+
+
+@pytest.mark.parametrize("softening", [0, 1])
+def test_periodic_wrap_across_the_box(gpu_ctx, softening):
+    """Two leaves on opposite faces of a periodic box, within the cutoff of each
+    other only through the minimum image."""
+    box = 40.0
+    g = np.random.default_rng(5)
+    a0 = np.array([0.5, 20.0, 20.0]) + (g.random((30, 3)) - 0.5) * 0.8
+    a1 = np.array([box - 0.8, 20.3, 19.6]) + (g.random((45, 3)) - 0.5) * 0.8
+    pos = np.concatenate([a0, a1])
+    ip = np.array([0, 30], np.int32)
+    npl = np.array([30, 45], np.int32)
+    tt = np.array([0, 1, 0, 1], np.int32)
+    ts = np.array([1, 0, 0, 1], np.int32)
+    ref = _oracle(pos, ip, npl, tt, ts, softening=softening, box=box)
+    a = _run(gpu_ctx, pos, ip, npl, tt, ts, softening=softening, box=box)
+    _check(a, ref)
+    # without the box the two leaves are ~39 apart, far beyond the cutoff: only self pairs act
+    cross = _oracle(pos, ip, npl, tt[:2], ts[:2], softening=softening, box=box)
+    assert np.all(np.linalg.norm(cross, axis=1) > 0)
+
+
+def test_bad_input_is_rejected_on_the_host(gpu_ctx):
+    pos, ip, npl = _leaves([4, 4], seed=6)
+    gpu_ctx.set_tuning()
+    gpu_ctx.set_law(pp.ForceLaw(eps=EPS, rs=RS))
+    gpu_ctx.set_particles(pos, ip, npl)
+    with pytest.raises(pp.P2PError):
+        gpu_ctx.set_tasks(np.array([0, 2], np.int32), np.array([1, 0], np.int32))      # leaf 2 does not exist
+    with pytest.raises(pp.P2PError):
+        gpu_ctx.set_tasks(np.array([-1], np.int32), np.array([0], np.int32))
+    with pytest.raises(pp.P2PError):
+        gpu_ctx.set_particles(pos, np.array([0, 6], np.int32), npl)                       # leaf runs past N
+    with pytest.raises(pp.P2PError):
+        gpu_ctx.set_law(pp.ForceLaw(eps=1e-9, rs=RS))                                     # below the table range
+    with pytest.raises(pp.P2PError):
+        gpu_ctx.set_law(pp.ForceLaw(eps=20.0, rs=RS))                                     # softening beyond the cutoff
+    # the context still works after the rejections
+    tt = np.array([0, 1], np.int32)
+    ts = np.array([1, 0], np.int32)
+    ref = _oracle(pos, ip, npl, tt, ts)
+    a = _run(gpu_ctx, pos, ip, npl, tt, ts)
+    _check(a, ref)
