@@ -81,7 +81,7 @@ def test_self_pairs_only(gpu_ctx):
 def test_repeated_leaf_pairs_count_twice(gpu_ctx):
     """A leaf pair listed twice contributes twice, as the reference's loop over
     its task list does."""
-    pos, ip, npl = _leaves([40, 25, 60], seed=4)
+    pos, ip, npl = _leaves([40, 25, 60], seed=4, spread=0.3)        # all within the cutoff
     tt = np.array([0, 0, 1, 2, 0], np.int32)
     ts = np.array([1, 1, 2, 0, 2], np.int32)
     ref = _oracle(pos, ip, npl, tt, ts)
