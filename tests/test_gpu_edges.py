@@ -107,7 +107,8 @@ def test_periodic_wrap_across_the_box(gpu_ctx, softening):
     ref = _oracle(pos, ip, npl, tt, ts, softening=softening, box=box)
     a = _run(gpu_ctx, pos, ip, npl, tt, ts, softening=softening, box=box)
     _check(a, ref)
-    # without the box the two leaves are ~39 apart, far beyond the cutoff: only self pairs act
+    # the leaves are ~39 apart inside the box, far beyond the cutoff: the cross
+    # pairs act only through the minimum image
     cross = _oracle(pos, ip, npl, tt[:2], ts[:2], softening=softening, box=box)
     assert np.all(np.linalg.norm(cross, axis=1) > 0)
 
