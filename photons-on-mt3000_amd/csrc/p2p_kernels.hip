@@ -14,8 +14,6 @@
 //
 // Reference semantics (what is computed, not how): P2P_core / P2P_core_1 /
 // P2P_kernel in src/gpu_kernel.impl.h:50-157, :260-362, :364-437.
-#include <type_traits>
-
 #include "p2p_device.h"
 
 namespace pp2p {
@@ -167,7 +165,6 @@ __device__ __forceinline__ void process_item(const ForceArgs &a, const LawConst 
     const int q = lane - phase * L;
     const bool live = phase < G;
     const int rphase = live ? phase : 0;            // idle lanes shadow phase 0 with dummy i-slots
-    const int glog = (G & (G - 1)) == 0 ? 31 - __builtin_clz(G) : -1;   // G = 2^glog, or -1
     const int nb = G * (kWave / G);                 // batch: the largest multiple of G <= 64
     const unsigned long long below = (1ull << lane) - 1ull;
 
@@ -245,36 +242,16 @@ __device__ __forceinline__ void process_item(const ForceArgs &a, const LawConst 
             }
         }
     };
-    // one batch of n (a multiple of G) tile entries.  With G = 2^GL a template
-    // constant the tile addresses of an unrolled step are immediate offsets of
-    // one pointer; other G step the pointer by G.
-    auto batch_g = [&](auto glc, int n) {
-        constexpr int GL = decltype(glc)::value;
-        const int ch = n >> GL;
-        const float4 *tp = tile + rphase;
-#pragma unroll 2
-        for (int u = 0; u < ch; ++u)
-            pairs_of(tp[u << GL]);
-    };
-    auto batch_any = [&](int n) {
+    // one batch of n (a multiple of G) tile entries, the pointer stepping by
+    // G.  One loop serves every phase count: specialised copies per count
+    // measured slower, the kernel outgrowing the instruction cache.
+    auto run_batch = [&](int n) {
         const int ch = n / G;
         const float4 *tp = tile + rphase;
-#pragma unroll 2
+#pragma unroll 4
         for (int u = 0; u < ch; ++u) {
             pairs_of(*tp);
             tp += G;
-        }
-    };
-    auto run_batch = [&](int n) {
-        switch (glog) {
-        case 0: batch_g(std::integral_constant<int, 0>{}, n); break;
-        case 1: batch_g(std::integral_constant<int, 1>{}, n); break;
-        case 2: batch_g(std::integral_constant<int, 2>{}, n); break;
-        case 3: batch_g(std::integral_constant<int, 3>{}, n); break;
-        case 4: batch_g(std::integral_constant<int, 4>{}, n); break;
-        case 5: batch_g(std::integral_constant<int, 5>{}, n); break;
-        case 6: batch_g(std::integral_constant<int, 6>{}, n); break;
-        default: batch_any(n); break;
         }
         if (FOLD64) {
 #pragma unroll
