@@ -23,6 +23,8 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <memory>
+#include <utility>
 #include <vector>
 
 #include <omp.h>
@@ -45,12 +47,31 @@ struct Leaf {
     float center[3];
 };
 
+// Allocator that leaves new elements uninitialised, so that large buffers are
+// first written (and their pages first touched) by the parallel loops that
+// fill them rather than by a serial zero fill.
+template <class T>
+struct UninitAlloc : std::allocator<T> {
+    template <class U>
+    struct rebind {
+        using other = UninitAlloc<U>;
+    };
+    UninitAlloc() = default;
+    template <class U>
+    UninitAlloc(const UninitAlloc<U> &) noexcept {}
+    template <class U>
+    void construct(U *) noexcept {}
+    template <class U, class... A>
+    void construct(U *p, A &&...a) { ::new ((void *)p) U(std::forward<A>(a)...); }
+};
+using IntBuf = std::vector<int, UninitAlloc<int>>;
+
 struct Tree {
     long long n = 0;
     double int2pos = 0.0;
     int maxleaf = 0;
-    std::vector<int> x[3];   // coordinate planes (permuted in place)
-    std::vector<int> idx;    // original particle index
+    IntBuf x[3];             // coordinate planes (permuted in place)
+    IntBuf idx;              // original particle index
     std::vector<Node> node;
     std::vector<Leaf> leaf;
 };
@@ -388,7 +409,10 @@ void expand(const Tree &t, float open_angle, float cutoff, std::vector<std::pair
 
 struct phost_lists {
     Tree tree;
-    std::vector<int> task_t, task_s;
+    // task pairs per walk frame, in frame order; off[k] = tasks before frame k
+    std::vector<std::vector<int>> ft, fs;
+    std::vector<long long> off;
+    int nthreads = 1;
 };
 
 extern "C" {
@@ -409,6 +433,7 @@ phost_lists *phost_build(long long n, const int *posi, const int *tag, double in
     for (int d = 0; d < 3; ++d)
         t.x[d].resize(n);
     t.idx.resize(n);
+#pragma omp parallel for schedule(static) num_threads(nthreads)
     for (long long i = 0; i < n; ++i) {
         for (int d = 0; d < 3; ++d)
             t.x[d][i] = posi[3 * i + d];
@@ -426,25 +451,22 @@ phost_lists *phost_build(long long n, const int *posi, const int *tag, double in
                 break;
             }
     }
-    // walk: expand the top into frames, walk each frame in parallel, then
-    // concatenate in frame order (= the sequential depth-first order)
+    // walk: expand the top into frames and walk each frame in parallel; the
+    // frames in order are the sequential depth-first order, and the per-frame
+    // lists are copied out at their prefix offsets by phost_get_tasks
     std::vector<std::pair<int, int>> frames{{0, 0}};
     expand(t, open_angle, cutoff_radius, frames, (size_t)nthreads * 64);
-    std::vector<std::vector<int>> ft(frames.size()), fs(frames.size());
+    h->ft.resize(frames.size());
+    h->fs.resize(frames.size());
 #pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
     for (long long k = 0; k < (long long)frames.size(); ++k) {
-        Walker w{t, open_angle, cutoff_radius, &ft[k], &fs[k]};
+        Walker w{t, open_angle, cutoff_radius, &h->ft[k], &h->fs[k]};
         w.walk(frames[k].first, frames[k].second);
     }
-    size_t total = 0;
-    for (auto &v : ft)
-        total += v.size();
-    h->task_t.reserve(total);
-    h->task_s.reserve(total);
-    for (size_t k = 0; k < frames.size(); ++k) {
-        h->task_t.insert(h->task_t.end(), ft[k].begin(), ft[k].end());
-        h->task_s.insert(h->task_s.end(), fs[k].begin(), fs[k].end());
-    }
+    h->off.assign(frames.size() + 1, 0);
+    for (size_t k = 0; k < frames.size(); ++k)
+        h->off[k + 1] = h->off[k] + (long long)h->ft[k].size();
+    h->nthreads = nthreads;
     return h;
 }
 
@@ -456,7 +478,7 @@ long long phost_count(const phost_lists *h, int what)
     case PHOST_NPART: return h->tree.n;
     case PHOST_NLEAF: return (long long)h->tree.leaf.size();
     case PHOST_NNODE: return (long long)h->tree.node.size();
-    case PHOST_NTASK: return (long long)h->task_t.size();
+    case PHOST_NTASK: return h->off.back();
     default: return -1;
     }
 }
@@ -482,8 +504,11 @@ void phost_get_leaves(const phost_lists *h, int *ip, int *np, int *active, float
 
 void phost_get_tasks(const phost_lists *h, int *task_t, int *task_s)
 {
-    std::memcpy(task_t, h->task_t.data(), sizeof(int) * h->task_t.size());
-    std::memcpy(task_s, h->task_s.data(), sizeof(int) * h->task_s.size());
+#pragma omp parallel for schedule(dynamic, 1) num_threads(h->nthreads)
+    for (long long k = 0; k < (long long)h->ft.size(); ++k) {
+        std::memcpy(task_t + h->off[k], h->ft[k].data(), sizeof(int) * h->ft[k].size());
+        std::memcpy(task_s + h->off[k], h->fs[k].data(), sizeof(int) * h->fs[k].size());
+    }
 }
 
 void phost_free(phost_lists *h) { delete h; }
