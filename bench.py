@@ -192,6 +192,8 @@ def main():
     ap.add_argument("--compare-plain", action="store_true",
                     help="also time the one-launch-per-target form the same way and report the ratio")
     ap.add_argument("--chunk-model", type=int, default=0, help="i-chunking: 0 default, -1 chunks of 64, >0 cost")
+    ap.add_argument("--no-device-walk", action="store_true",
+                    help="skip building the list again by the device walk (tree workloads)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -344,6 +346,25 @@ def main():
             log(f"[bench] active {f}: {nact} active, {ent['ntask']} tasks, N_list {sf['npairs']:.3e}, "
                 f"{f_ms:.3f} ms/step ({f_ms / ms_step:.3f} of the all-active step), prepare {p_ms:.1f} ms"
                 + (f", p99 {ent['rel_err']['p99']:.2e}" if "rel_err" in ent else ""))
+    lists = None
+    if "tree" in w.meta and args.active <= 0 and not args.no_device_walk:
+        # the same list built on the device from the host tree (best of 3 calls,
+        # tree upload included), checked pair for pair against the host walk
+        m = w.meta
+        dts = []
+        for _ in range(3):
+            t1 = time.perf_counter()
+            nwalk = ctx.walk_tasks(m["tree"], m["active_leaf"], np.float32(m["open_angle"]),
+                                   np.float32(m["cutoff_radius"]))
+            dts.append(time.perf_counter() - t1)
+        tt, ts = ctx.get_tasks()
+        same = bool(np.array_equal(tt, w.task_t) and np.array_equal(ts, w.task_s))
+        del tt, ts
+        ls = m["list_seconds"]
+        lists = {"host_tree_s": ls["tree"], "host_walk_s": ls["walk"], "host_threads": threads,
+                 "device_walk_ms": 1e3 * min(dts), "ntask": int(nwalk), "identical_to_host": same}
+        log(f"[bench] lists: host tree {ls['tree']:.2f}s, host walk {ls['walk']:.2f}s on {threads} threads; "
+            f"device walk {1e3 * min(dts):.1f} ms ({nwalk} tasks, identical to the host walk: {same})")
     ctx.close()
 
     copy_cal = None
@@ -387,6 +408,7 @@ def main():
         "grid_form": grid_form,
         "active_sweep": sweep,
         "copy_calibration": copy_cal,
+        "lists": lists,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS,
                      "traffic": MEASURED_TRAFFIC.get(w.name, {}).get("bytes") if args.active <= 0 else None,

@@ -140,6 +140,27 @@ int pp2p_set_active(pp2p_ctx *ctx, const unsigned char *act, int active_level);
 int pp2p_set_tasks(pp2p_ctx *ctx, long long ntask, const int *task_t, const int *task_s);
 int pp2p_add_tasks(pp2p_ctx *ctx, long long ntask, const int *task_t, const int *task_s);
 
+/*
+ * Interaction list built on the device from a kd-tree: the symmetric dual
+ * walk of src/fmm.c:656-792 (acceptance src/fmm.c:411-479, targets marked as
+ * src/fmm.c:1226-1236), run level by level.  It replaces the context's list
+ * with the walk's (target, source) leaf pairs, 0-based, in the order the
+ * reference's recursive walk emits them, so the list equals what the host
+ * walk would pass to pp2p_set_tasks.  Tree: nnode nodes, node k with sons
+ * node_son[2k], node_son[2k+1] (>= 0: node id, which must be > k; < 0: leaf
+ * -(id + 1)), root = node 0 (or the single leaf when nnode == 0); centres and
+ * widths of the node and leaf boxes as nnode x 3 / nleaf x 3 FP32
+ * (src/fmm.c:273-327); leaf_active[l] >= 0 marks a target leaf.  nleaf is
+ * the context's (pp2p_set_particles_*).  *ntask (nullable) receives the list
+ * length.  Synchronous.
+ */
+int pp2p_walk_tasks(pp2p_ctx *ctx, int nnode, const int *node_son, const float *node_center,
+                    const float *node_width, const float *leaf_center, const float *leaf_width,
+                    const int *leaf_active, float open_angle, float cutoff_radius, long long *ntask);
+/* Copies the context's current list out (each pointer nullable); *ntask
+ * (nullable) receives its length.  Synchronous. */
+int pp2p_get_tasks(pp2p_ctx *ctx, long long *ntask, int *task_t, int *task_s);
+
 /* Builds the target CSR and the cost-sorted work queue of the current list
  * on the device.  pp2p_compute calls it when the list changed. */
 int pp2p_prepare(pp2p_ctx *ctx);

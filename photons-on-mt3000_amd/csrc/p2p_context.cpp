@@ -14,6 +14,7 @@
 #include "forcelaw.h"
 #include "p2p_build.h"
 #include "p2p_launch.h"
+#include "p2p_walk.h"
 
 using namespace pp2p;
 
@@ -121,6 +122,21 @@ struct pp2p_ctx {
     DevBuf<unsigned long long> pair_count;
     DevBuf<unsigned char> temp;
 
+    // device walk: the tree, and per round its frames, codes, child offsets
+    // and task offsets (kept for the backward and position passes)
+    struct WalkRound {
+        DevBuf<int2> f;
+        DevBuf<unsigned char> code;
+        DevBuf<unsigned> off;
+        DevBuf<long long> x;
+        long long n = 0;
+    };
+    DevBuf<int2> wson;
+    DevBuf<float4> wnc, wnw, wlc, wlw;
+    DevBuf<int> wact;
+    std::vector<WalkRound> wr;
+    DevBuf<long long> wcnt, wpos_a, wpos_b;
+
     float last_kernel_ms = 0.f, last_prepare_ms = 0.f;
     bool kernel_timed = false, prepare_timed = false;
 };
@@ -198,6 +214,11 @@ void pp2p_destroy(pp2p_ctx *c)
     c->flag.release(); c->rowidx.release(); c->row_tgt.release(); c->row_off.release(); c->row_cost.release();
     c->total.release(); c->item_count.release(); c->item_off.release(); c->items_rows.release();
     c->items_q.release(); c->cost.release(); c->cost_sorted.release();
+    c->wson.release(); c->wnc.release(); c->wnw.release(); c->wlc.release(); c->wlw.release(); c->wact.release();
+    for (auto &r : c->wr) {
+        r.f.release(); r.code.release(); r.off.release(); r.x.release();
+    }
+    c->wcnt.release(); c->wpos_a.release(); c->wpos_b.release();
     c->idx.release(); c->idx_sorted.release(); c->counter.release(); c->temp.release();
     c->pair_count.release();
     if (c->ev_k0) (void)hipEventDestroy(c->ev_k0);
@@ -437,6 +458,152 @@ int pp2p_add_tasks(pp2p_ctx *c, long long n, const int *tt, const int *ts)
     if (!c->have_particles)
         return fail(PP2P_ERR_STATE, "set_particles before add_tasks");
     return append_tasks(c, n, tt, ts);
+}
+
+static int upload_boxes(pp2p_ctx *c, DevBuf<float4> &dst, int n, const float *xyz)
+{
+    std::vector<float4> h((size_t)std::max(n, 1));
+    for (int k = 0; k < n; ++k)
+        h[k] = make_float4(xyz[3 * k], xyz[3 * k + 1], xyz[3 * k + 2], 0.f);
+    HIPCHK(dst.reserve(h.size()));
+    HIPCHK(hipMemcpyAsync(dst.p, h.data(), sizeof(float4) * h.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return PP2P_OK;
+}
+
+int pp2p_walk_tasks(pp2p_ctx *c, int nnode, const int *node_son, const float *node_center, const float *node_width,
+                    const float *leaf_center, const float *leaf_width, const int *leaf_active, float open_angle,
+                    float cutoff_radius, long long *ntask)
+{
+    int st = check_ctx(c);
+    if (st)
+        return st;
+    if (!c->have_particles)
+        return fail(PP2P_ERR_STATE, "set_particles before walk_tasks");
+    const int nleaf = c->nleaf;
+    if (nnode < 0 || (nnode > 0 && (!node_son || !node_center || !node_width)) || !leaf_center || !leaf_width ||
+        !leaf_active)
+        return fail(PP2P_ERR_ARG, "invalid tree arrays");
+    if (nnode == 0 && nleaf != 1)
+        return fail(PP2P_ERR_ARG, "a tree without nodes holds exactly one leaf");
+    if (!(open_angle >= 0.f) || !(cutoff_radius >= 0.f) || std::isinf(cutoff_radius))
+        return fail(PP2P_ERR_ARG, "invalid open_angle or cutoff_radius");
+    // sons are created after their parent (ids increase down the tree), so
+    // every round moves at least one cell of each opened frame one level down
+    // and the walk ends within 2 * nnode + 2 rounds
+    for (int k = 0; k < nnode; ++k)
+        for (int h = 0; h < 2; ++h) {
+            const int s = node_son[2 * k + h];
+            if (s >= 0 ? (s <= k || s >= nnode) : (-(long long)s - 1 >= nleaf))
+                return fail(PP2P_ERR_ARG, "node " + std::to_string(k) + " has an invalid son");
+        }
+    HIPCHK(c->wson.reserve(std::max(nnode, 1)));
+    if (nnode)
+        HIPCHK(hipMemcpyAsync(c->wson.p, node_son, sizeof(int2) * nnode, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c->wact.reserve(nleaf));
+    HIPCHK(hipMemcpyAsync(c->wact.p, leaf_active, sizeof(int) * nleaf, hipMemcpyHostToDevice, c->stream));
+    if ((st = upload_boxes(c, c->wnc, nnode, node_center)) || (st = upload_boxes(c, c->wnw, nnode, node_width)) ||
+        (st = upload_boxes(c, c->wlc, nleaf, leaf_center)) || (st = upload_boxes(c, c->wlw, nleaf, leaf_width)))
+        return st;
+    WalkTree t{c->wson.p, c->wnc.p, c->wnw.p, c->wlc.p, c->wlw.p, c->wact.p, open_angle, cutoff_radius};
+
+    c->ntask = 0;
+    c->csr_ready = c->items_ready = false;
+    auto &wr = c->wr;
+    // forward: the frames of each round
+    if (wr.empty())
+        wr.resize(1);
+    HIPCHK(wr[0].f.reserve(1));
+    const int2 root = nnode > 0 ? make_int2(0, 0) : make_int2(-1, -1);
+    HIPCHK(hipMemcpyAsync(wr[0].f.p, &root, sizeof(int2), hipMemcpyHostToDevice, c->stream));
+    wr[0].n = 1;
+    int last = 0;
+    for (int r = 0;; ++r) {
+        if (r > 2 * nnode + 2)
+            return fail(PP2P_ERR_ARG, "tree walk did not end");
+        const long long n = wr[r].n;
+        // child offsets are 32-bit: a round holds at most 2^29 frames
+        if (n > (1ll << 29))
+            return fail(PP2P_ERR_ARG, "tree walk round larger than 2^29 frames");
+        size_t tb = 0;
+        HIPCHK(walk_code(t, wr[r].f.p, n, nullptr, nullptr, nullptr, &tb, c->stream));
+        if ((st = ensure_temp(c, tb)))
+            return st;
+        tb = c->temp.cap;
+        HIPCHK(wr[r].code.reserve(n));
+        HIPCHK(wr[r].off.reserve(n + 1));
+        HIPCHK(walk_code(t, wr[r].f.p, n, wr[r].code.p, wr[r].off.p, c->temp.p, &tb, c->stream));
+        unsigned nn = 0;
+        HIPCHK(hipMemcpyAsync(&nn, wr[r].off.p + n, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        if (nn == 0) {
+            last = r;
+            break;
+        }
+        if ((int)wr.size() < r + 2)
+            wr.resize(r + 2);
+        HIPCHK(wr[r + 1].f.reserve(nn));
+        HIPCHK(walk_children(t, wr[r].f.p, n, wr[r].off.p, wr[r + 1].f.p, c->stream));
+        wr[r + 1].n = nn;
+    }
+    // backward: tasks at and below each frame, last round first (every
+    // buffer the passes share is sized before the first launch)
+    long long nmax = 1;
+    size_t tmax = 0;
+    for (int r = 0; r <= last; ++r) {
+        size_t tb = 0;
+        HIPCHK(walk_below(nullptr, nullptr, nullptr, wr[r].n, nullptr, nullptr, nullptr, &tb, c->stream));
+        tmax = std::max(tmax, tb);
+        nmax = std::max(nmax, wr[r].n);
+        HIPCHK(wr[r].x.reserve(wr[r].n + 1));
+    }
+    if ((st = ensure_temp(c, tmax)))
+        return st;
+    HIPCHK(c->wcnt.reserve(nmax));
+    HIPCHK(c->wpos_a.reserve(nmax));
+    HIPCHK(c->wpos_b.reserve(nmax));
+    for (int r = last; r >= 0; --r) {
+        const long long *xn = r < last ? wr[r + 1].x.p : nullptr;
+        size_t tb = c->temp.cap;
+        HIPCHK(walk_below(wr[r].code.p, wr[r].off.p, xn, wr[r].n, c->wcnt.p, wr[r].x.p, c->temp.p, &tb, c->stream));
+    }
+    long long total = 0;
+    HIPCHK(hipMemcpyAsync(&total, wr[0].x.p + 1, sizeof(long long), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (total > (1ll << 31) - 1)
+        return fail(PP2P_ERR_ARG, "more than 2^31-1 tasks in one list");
+    // positions: first task of each frame, round by round; emitting frames write
+    HIPCHK(c->task_t.reserve(std::max(total, 1ll)));
+    HIPCHK(c->task_s.reserve(std::max(total, 1ll)));
+    const long long zero = 0;
+    HIPCHK(hipMemcpyAsync(c->wpos_a.p, &zero, sizeof(long long), hipMemcpyHostToDevice, c->stream));
+    DevBuf<long long> *pc = &c->wpos_a, *pn = &c->wpos_b;
+    for (int r = 0; r <= last; ++r) {
+        const long long *xn = r < last ? wr[r + 1].x.p : nullptr;
+        HIPCHK(walk_emit(wr[r].f.p, wr[r].code.p, wr[r].off.p, xn, wr[r].n, pc->p, r < last ? pn->p : nullptr,
+                         c->task_t.p, c->task_s.p, c->stream));
+        std::swap(pc, pn);
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->ntask = total;
+    if (ntask)
+        *ntask = c->ntask;
+    return PP2P_OK;
+}
+
+int pp2p_get_tasks(pp2p_ctx *c, long long *ntask, int *task_t, int *task_s)
+{
+    int st = check_ctx(c);
+    if (st)
+        return st;
+    if (ntask)
+        *ntask = c->ntask;
+    if (c->ntask && task_t)
+        HIPCHK(hipMemcpyAsync(task_t, c->task_t.p, sizeof(int) * c->ntask, hipMemcpyDeviceToHost, c->stream));
+    if (c->ntask && task_s)
+        HIPCHK(hipMemcpyAsync(task_s, c->task_s.p, sizeof(int) * c->ntask, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return PP2P_OK;
 }
 
 static int build_csr(pp2p_ctx *c)

@@ -417,15 +417,16 @@ struct phost_lists {
 
 extern "C" {
 
-phost_lists *phost_build(long long n, const int *posi, const int *tag, double int2pos, int maxleaf,
-                         float open_angle, float cutoff_radius, const float bdl[3], const float bdr[3],
-                         int nthreads)
+phost_lists *phost_build_tree(long long n, const int *posi, const int *tag, double int2pos, int maxleaf,
+                              const float bdl[3], const float bdr[3], int nthreads)
 {
     if (n <= 0 || !posi || !tag || maxleaf < 1 || !(int2pos > 0.0))
         return nullptr;
     if (nthreads <= 0)
         nthreads = omp_get_max_threads();
     phost_lists *h = new phost_lists();
+    h->nthreads = nthreads;
+    h->off.assign(1, 0);
     Tree &t = h->tree;
     t.n = n;
     t.int2pos = int2pos;
@@ -451,13 +452,23 @@ phost_lists *phost_build(long long n, const int *posi, const int *tag, double in
                 break;
             }
     }
-    // walk: expand the top into frames and walk each frame in parallel; the
-    // frames in order are the sequential depth-first order, and the per-frame
-    // lists are copied out at their prefix offsets by phost_get_tasks
+    return h;
+}
+
+int phost_walk(phost_lists *h, float open_angle, float cutoff_radius, int nthreads)
+{
+    if (!h)
+        return -1;
+    if (nthreads <= 0)
+        nthreads = omp_get_max_threads();
+    const Tree &t = h->tree;
+    // expand the top into frames and walk each frame in parallel; the frames
+    // in order are the sequential depth-first order, and the per-frame lists
+    // are copied out at their prefix offsets by phost_get_tasks
     std::vector<std::pair<int, int>> frames{{0, 0}};
     expand(t, open_angle, cutoff_radius, frames, (size_t)nthreads * 64);
-    h->ft.resize(frames.size());
-    h->fs.resize(frames.size());
+    h->ft.assign(frames.size(), {});
+    h->fs.assign(frames.size(), {});
 #pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
     for (long long k = 0; k < (long long)frames.size(); ++k) {
         Walker w{t, open_angle, cutoff_radius, &h->ft[k], &h->fs[k]};
@@ -467,6 +478,16 @@ phost_lists *phost_build(long long n, const int *posi, const int *tag, double in
     for (size_t k = 0; k < frames.size(); ++k)
         h->off[k + 1] = h->off[k] + (long long)h->ft[k].size();
     h->nthreads = nthreads;
+    return 0;
+}
+
+phost_lists *phost_build(long long n, const int *posi, const int *tag, double int2pos, int maxleaf,
+                         float open_angle, float cutoff_radius, const float bdl[3], const float bdr[3],
+                         int nthreads)
+{
+    phost_lists *h = phost_build_tree(n, posi, tag, int2pos, maxleaf, bdl, bdr, nthreads);
+    if (h)
+        phost_walk(h, open_angle, cutoff_radius, nthreads);
     return h;
 }
 
@@ -498,6 +519,19 @@ void phost_get_leaves(const phost_lists *h, int *ip, int *np, int *active, float
         for (int d = 0; d < 3; ++d) {
             if (center) center[3 * l + d] = L[l].center[d];
             if (width) width[3 * l + d] = L[l].width[d];
+        }
+    }
+}
+
+void phost_get_nodes(const phost_lists *h, int *son, float *center, float *width)
+{
+    const auto &N = h->tree.node;
+    for (size_t k = 0; k < N.size(); ++k) {
+        for (int n = 0; n < 2; ++n)
+            if (son) son[2 * k + n] = N[k].son[n];
+        for (int d = 0; d < 3; ++d) {
+            if (center) center[3 * k + d] = N[k].center[d];
+            if (width) width[3 * k + d] = N[k].width[d];
         }
     }
 }

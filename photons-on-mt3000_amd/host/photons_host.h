@@ -18,11 +18,19 @@ typedef struct phost_lists phost_lists;
 phost_lists *phost_build(long long n, const int *posi, const int *tag, double int2pos, int maxleaf,
                          float open_angle, float cutoff_radius, const float bdl[3], const float bdr[3],
                          int nthreads);
+/* The same in two steps: the tree alone (boxes and target marking included,
+ * no tasks), then the walk over it (replaces the tasks; 0 on success). */
+phost_lists *phost_build_tree(long long n, const int *posi, const int *tag, double int2pos, int maxleaf,
+                              const float bdl[3], const float bdr[3], int nthreads);
+int phost_walk(phost_lists *h, float open_angle, float cutoff_radius, int nthreads);
 long long phost_count(const phost_lists *h, int what);
 /* perm[k] = original index of the particle at tree position k */
 void phost_get_perm(const phost_lists *h, int *perm);
 /* any output pointer may be NULL; center/width are nleaf x 3 */
 void phost_get_leaves(const phost_lists *h, int *ip, int *np, int *active, float *center, float *width);
+/* any output pointer may be NULL; son is nnode x 2 (>= 0 node id, < 0 leaf
+ * -(id + 1)), center/width nnode x 3 */
+void phost_get_nodes(const phost_lists *h, int *son, float *center, float *width);
 void phost_get_tasks(const phost_lists *h, int *task_t, int *task_s);
 void phost_free(phost_lists *h);
 

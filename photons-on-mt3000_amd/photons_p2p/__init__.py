@@ -59,6 +59,7 @@ CORE_SYMBOLS = (
     "pp2p_version", "pp2p_device_count", "pp2p_last_error", "pp2p_create", "pp2p_destroy",
     "pp2p_set_params", "pp2p_set_tuning", "pp2p_get_tuning", "pp2p_set_particles_f64",
     "pp2p_set_particles_i32", "pp2p_set_active", "pp2p_set_tasks", "pp2p_add_tasks",
+    "pp2p_walk_tasks", "pp2p_get_tasks",
     "pp2p_prepare", "pp2p_compute", "pp2p_zero_acc", "pp2p_run_graph", "pp2p_sync",
     "pp2p_get_acc_f64", "pp2p_get_acc_f32_soa", "pp2p_list_stats", "pp2p_last_kernel_ms",
     "pp2p_last_prepare_ms", "pp2p_count_inside", "pp2p_time_steps", "pp2p_law_eval",
@@ -93,6 +94,9 @@ def load(path: str | None = None) -> ctypes.CDLL:
         "pp2p_set_active": (i32, [vp, vp, i32]),
         "pp2p_set_tasks": (i32, [vp, i64, vp, vp]),
         "pp2p_add_tasks": (i32, [vp, i64, vp, vp]),
+        "pp2p_walk_tasks": (i32, [vp, i32, vp, vp, vp, vp, vp, vp, ctypes.c_float, ctypes.c_float,
+                                  ctypes.POINTER(i64)]),
+        "pp2p_get_tasks": (i32, [vp, ctypes.POINTER(i64), vp, vp]),
         "pp2p_prepare": (i32, [vp]),
         "pp2p_compute": (i32, [vp, i32]),
         "pp2p_zero_acc": (i32, [vp]),
@@ -237,6 +241,34 @@ class Context:
         if tt.shape != ts.shape:
             raise ValueError("task_t and task_s differ in length")
         self._chk(self._lib.pp2p_add_tasks(self._h, tt.size, _ptr(tt), _ptr(ts)))
+
+    def walk_tasks(self, tree: dict, active: np.ndarray, open_angle: float, cutoff_radius: float) -> int:
+        """Builds the list on the device by the dual walk of `tree` (node_son,
+        node_center, node_width, leaf_center, leaf_width as build_lists
+        returns them) and makes it the context's list.  Returns its length."""
+        son = np.ascontiguousarray(tree["node_son"], np.int32)
+        nc = np.ascontiguousarray(tree["node_center"], np.float32)
+        nw = np.ascontiguousarray(tree["node_width"], np.float32)
+        lc = np.ascontiguousarray(tree["leaf_center"], np.float32)
+        lw = np.ascontiguousarray(tree["leaf_width"], np.float32)
+        act = np.ascontiguousarray(active, np.int32)
+        if lc.shape != (self.nleaf, 3) or lw.shape != lc.shape or act.shape != (self.nleaf,):
+            raise ValueError("leaf arrays do not match the context's leaves")
+        if son.ndim != 2 or son.shape[1] != 2 or nc.shape != (son.shape[0], 3) or nw.shape != nc.shape:
+            raise ValueError("node arrays must be nnode x 2 and nnode x 3")
+        n = ctypes.c_longlong(0)
+        self._chk(self._lib.pp2p_walk_tasks(self._h, son.shape[0], _ptr(son), _ptr(nc), _ptr(nw), _ptr(lc),
+                                            _ptr(lw), _ptr(act), float(open_angle), float(cutoff_radius),
+                                            ctypes.byref(n)))
+        return n.value
+
+    def get_tasks(self) -> tuple[np.ndarray, np.ndarray]:
+        n = ctypes.c_longlong(0)
+        self._chk(self._lib.pp2p_get_tasks(self._h, ctypes.byref(n), None, None))
+        tt = np.empty(n.value, np.int32)
+        ts = np.empty(n.value, np.int32)
+        self._chk(self._lib.pp2p_get_tasks(self._h, ctypes.byref(n), _ptr(tt), _ptr(ts)))
+        return tt, ts
 
     # -- execution -----------------------------------------------------------
     def prepare(self) -> float:

@@ -175,32 +175,48 @@ def _host_lib():
     lib.phost_build.restype = vp
     lib.phost_build.argtypes = [ctypes.c_longlong, vp, vp, ctypes.c_double, ctypes.c_int, ctypes.c_float,
                                 ctypes.c_float, vp, vp, ctypes.c_int]
+    lib.phost_build_tree.restype = vp
+    lib.phost_build_tree.argtypes = [ctypes.c_longlong, vp, vp, ctypes.c_double, ctypes.c_int, vp, vp, ctypes.c_int]
+    lib.phost_walk.restype = ctypes.c_int
+    lib.phost_walk.argtypes = [vp, ctypes.c_float, ctypes.c_float, ctypes.c_int]
     lib.phost_count.restype = ctypes.c_longlong
     lib.phost_count.argtypes = [vp, ctypes.c_int]
     lib.phost_get_perm.argtypes = [vp, vp]
     lib.phost_get_leaves.argtypes = [vp, vp, vp, vp, vp, vp]
+    lib.phost_get_nodes.argtypes = [vp, vp, vp, vp]
     lib.phost_get_tasks.argtypes = [vp, vp, vp]
     lib.phost_free.argtypes = [vp]
     return lib
 
 
 def build_lists(posi: np.ndarray, tag: np.ndarray, int2pos: float, maxleaf: int, open_angle: float,
-                cutoff_radius: float, bdl: np.ndarray, bdr: np.ndarray, nthreads: int = 0) -> dict:
+                cutoff_radius: float, bdl: np.ndarray, bdr: np.ndarray, nthreads: int = 0,
+                walk: bool = True, with_tree: bool = False) -> dict:
     """kd-tree + dual walk (libphotons_host.so).  Returns perm (tree order ->
-    input index), leaf_ip, leaf_np, active, task_t, task_s (0-based leaf ids)."""
+    input index), leaf_ip, leaf_np, active, task_t, task_s (0-based leaf ids)
+    and the host seconds of the two steps (tree_s, walk_s).  walk=False skips
+    the walk (empty task arrays); with_tree adds the tree as the device walk
+    reads it: node_son (nnode x 2), node_center, node_width, leaf_center,
+    leaf_width (x 3, FP32)."""
+    import time
     lib = _host_lib()
     P = lambda a: a.ctypes.data_as(__import__("ctypes").c_void_p)  # noqa: E731
     posi = np.ascontiguousarray(posi, np.int32)
     tag = np.ascontiguousarray(tag, np.int32)
     bdl = np.ascontiguousarray(bdl, np.float32)
     bdr = np.ascontiguousarray(bdr, np.float32)
-    h = lib.phost_build(posi.shape[0], P(posi), P(tag), int2pos, maxleaf, open_angle, cutoff_radius,
-                        P(bdl), P(bdr), nthreads)
+    t0 = time.perf_counter()
+    h = lib.phost_build_tree(posi.shape[0], P(posi), P(tag), int2pos, maxleaf, P(bdl), P(bdr), nthreads)
     if not h:
         raise ValueError("phost_build rejected its input")
+    t1 = time.perf_counter()
     try:
+        if walk and lib.phost_walk(h, open_angle, cutoff_radius, nthreads) != 0:
+            raise ValueError("phost_walk failed")
+        t2 = time.perf_counter()
         n = lib.phost_count(h, 0)
         nl = lib.phost_count(h, 1)
+        nn = lib.phost_count(h, 2)
         nt = lib.phost_count(h, 3)
         perm = np.empty(n, np.int32)
         ip = np.empty(nl, np.int32)
@@ -208,12 +224,21 @@ def build_lists(posi: np.ndarray, tag: np.ndarray, int2pos: float, maxleaf: int,
         act = np.empty(nl, np.int32)
         tt = np.empty(nt, np.int32)
         ts = np.empty(nt, np.int32)
+        out = {"perm": perm, "leaf_ip": ip, "leaf_np": npl, "active": act, "task_t": tt, "task_s": ts}
         lib.phost_get_perm(h, P(perm))
-        lib.phost_get_leaves(h, P(ip), P(npl), P(act), None, None)
+        if with_tree:
+            out.update(node_son=np.empty((nn, 2), np.int32), node_center=np.empty((nn, 3), np.float32),
+                       node_width=np.empty((nn, 3), np.float32), leaf_center=np.empty((nl, 3), np.float32),
+                       leaf_width=np.empty((nl, 3), np.float32))
+            lib.phost_get_leaves(h, P(ip), P(npl), P(act), P(out["leaf_center"]), P(out["leaf_width"]))
+            lib.phost_get_nodes(h, P(out["node_son"]), P(out["node_center"]), P(out["node_width"]))
+        else:
+            lib.phost_get_leaves(h, P(ip), P(npl), P(act), None, None)
         lib.phost_get_tasks(h, P(tt), P(ts))
     finally:
         lib.phost_free(h)
-    return {"perm": perm, "leaf_ip": ip, "leaf_np": npl, "active": act, "task_t": tt, "task_s": ts}
+    out.update(tree_s=t1 - t0, walk_s=t2 - t1 if walk else 0.0)
+    return out
 
 
 def tree_workload(name: str, pos: np.ndarray, box: float, mesh: int, maxleaf: int, open_angle: float,
@@ -241,13 +266,15 @@ def tree_workload(name: str, pos: np.ndarray, box: float, mesh: int, maxleaf: in
     if progress:
         progress(f"{allp.shape[0]} particles ({gpos.shape[0]} ghost copies) on the grid; building tree + lists")
     lists = build_lists(allp, tag, int2pos, maxleaf, np.float32(open_angle), np.float32(cutoff_radius),
-                        bdl, bdr, nthreads)
+                        bdl, bdr, nthreads, with_tree=True)
     perm = lists["perm"]
     m = dict(meta or {})
     m.update({"int2pos": int2pos, "box_i": box_i, "posi": allp[perm], "tag": tag[perm],
               "local_index": local_index[perm], "nlocal": posi.shape[0], "maxleaf": maxleaf,
               "open_angle": open_angle, "cutoff_radius": cutoff_radius, "mesh": mesh,
-              "bdl": bdl, "bdr": bdr, "active_leaf": lists["active"]})
+              "bdl": bdl, "bdr": bdr, "active_leaf": lists["active"],
+              "tree": {k: lists[k] for k in ("node_son", "node_center", "node_width", "leaf_center", "leaf_width")},
+              "list_seconds": {"tree": lists["tree_s"], "walk": lists["walk_s"], "threads": nthreads}})
     w = Workload(name, allp[perm].astype(np.float64) * int2pos, lists["leaf_ip"], lists["leaf_np"],
                  lists["task_t"], lists["task_s"], 0.0, rs, eps, softening=softening, meta=m)
     return w

@@ -105,3 +105,107 @@ def test_generators_seeded_and_shaped():
     assert h.shape == (4000, 3)
     u = W.uneven_pairs(seed=1, lattice=16, n_pool=300, max_src=50, reach=3, total_pairs=2000)
     assert u.task_t.size == 2000 and u.leaf_np.min() >= 1 and u.leaf_np.max() <= 256
+
+
+def _accept_np(wi, wj, dist, theta, rc):
+    """Vectorised acceptance test (src/fmm.c:411-479) in per-operation FP32."""
+    f32 = np.float32
+    w = ((wi + wj).astype(np.float64) * 0.5).astype(f32)
+    m = np.abs(dist) - w
+    m[m <= 0] = f32(0)
+    dd2 = (dist[:, 0] * dist[:, 0] + dist[:, 1] * dist[:, 1]) + dist[:, 2] * dist[:, 2]
+    dm2 = (m[:, 0] * m[:, 0] + m[:, 1] * m[:, 1]) + m[:, 2] * m[:, 2]
+    wmax = np.abs(np.maximum(np.maximum(w[:, 0], w[:, 1]), w[:, 2])) * f32(2)
+    r = np.where(wmax * wmax < (f32(theta) * f32(theta)) * dd2, 1, 0)
+    r = np.where(dm2 > f32(rc) * f32(rc), -1, r)
+    return np.where(((m[:, 0] + m[:, 1]) + m[:, 2]).astype(np.float64) < 1.0e-15, 0, r)
+
+
+def _walk_levels(tree, active, theta, rc):
+    """The walk run level by level (every frame replaced by its children in
+    call order, leaf-leaf pairs carried) instead of recursively."""
+    son, nc, nw = tree["node_son"], tree["node_center"], tree["node_width"]
+    lc, lw = tree["leaf_center"], tree["leaf_width"]
+    im, jm = np.zeros(1, np.int64), np.zeros(1, np.int64)
+    while True:
+        li, lj = im < 0, jm < 0
+        if np.all(li & lj):
+            keep = active[-im - 1] >= 0
+            return (-im[keep] - 1).astype(np.int32), (-jm[keep] - 1).astype(np.int32)
+        k = np.zeros(im.size, np.int64)
+        ch = np.zeros((im.size, 4, 2), np.int64)
+        ll = li & lj
+        k[ll] = (active[-im[ll] - 1] >= 0)
+        ch[ll, 0] = np.stack([im[ll], jm[ll]], 1)
+        sf = (im == jm) & ~li
+        s = son[im[sf]]
+        k[sf] = 4
+        ch[sf] = np.stack([s[:, [0, 0]], s[:, [0, 1]], s[:, [1, 0]], s[:, [1, 1]]], 1)
+        rest = ~ll & ~sf
+        ci = np.where(li[:, None], lc[np.maximum(-im - 1, 0)], nc[np.maximum(im, 0)])
+        cj = np.where(lj[:, None], lc[np.maximum(-jm - 1, 0)], nc[np.maximum(jm, 0)])
+        wi = np.where(li[:, None], lw[np.maximum(-im - 1, 0)], nw[np.maximum(im, 0)])
+        wj = np.where(lj[:, None], lw[np.maximum(-jm - 1, 0)], nw[np.maximum(jm, 0)])
+        opened = rest & (_accept_np(wi, wj, ci - cj, theta, rc) == 0)
+        wsi = (wi[:, 0] + wi[:, 1]) + wi[:, 2]
+        wsj = (wj[:, 0] + wj[:, 1]) + wj[:, 2]
+        split_i = opened & ~li & (lj | (wsi > wsj))
+        split_j = opened & ~split_i
+        si, sj = son[np.maximum(im, 0)], son[np.maximum(jm, 0)]
+        ch[split_i, 0] = np.stack([si[split_i, 0], jm[split_i]], 1)
+        ch[split_i, 1] = np.stack([si[split_i, 1], jm[split_i]], 1)
+        ch[split_j, 0] = np.stack([im[split_j], sj[split_j, 0]], 1)
+        ch[split_j, 1] = np.stack([im[split_j], sj[split_j, 1]], 1)
+        k[opened] = 2
+        nxt = ch[np.arange(4)[None, :] < k[:, None]]
+        im, jm = nxt[:, 0], nxt[:, 1]
+
+
+@pytest.mark.parametrize("n1,maxleaf,theta", [(24, 16, 0.5), (20, 64, 0.4), (16, 32, 0.5)])
+def test_level_synchronous_walk_matches_recursive(n1, maxleaf, theta):
+    """The order argument of the device walk, on the CPU: expanding every
+    frame into its children in call order, one level at a time, yields the
+    recursive walk's list in its order."""
+    g = np.random.default_rng(100 + n1)
+    box = 100.0
+    pos = np.mod(g.random((n1 ** 3, 3)) * box + 0.0, box)
+    pos[: pos.shape[0] // 4] = np.mod(30.0 + g.normal(0, 3.0, (pos.shape[0] // 4, 3)), box)
+    cr = 4.5 * 1.25 * box / n1
+    int2pos = W.int_grid(box, cr)
+    posi, box_i = W.to_grid(pos, box, int2pos)
+    gp, _ = W.ghost_images(posi, box_i, int(np.ceil(np.float32(cr) / int2pos)))
+    allp = np.concatenate([posi, gp])
+    tag = np.concatenate([np.ones(len(posi), np.int32), np.zeros(len(gp), np.int32)])
+    bdl, bdr = W.combine_bounds(allp, int2pos, box)
+    L = W.build_lists(allp, tag, int2pos, maxleaf, np.float32(theta), np.float32(cr), bdl, bdr, with_tree=True)
+    assert np.any(L["active"] < 0)                       # ghost-only leaves exist and are no targets
+    tt, ts = _walk_levels(L, L["active"], theta, cr)
+    np.testing.assert_array_equal(tt, L["task_t"])
+    np.testing.assert_array_equal(ts, L["task_s"])
+
+
+def test_tree_export_and_two_step_build():
+    f = np.load(GOLDEN[0]) if GOLDEN else None
+    w = W.config_plain(12)
+    m = w.meta
+    t = m["tree"]
+    son = t["node_son"]
+    nn = son.shape[0]
+    # sons are created after their parent; every leaf and every node but the root is a son exactly once
+    ids = np.arange(nn)[:, None]
+    assert np.all(np.where(son >= 0, son > ids, True))
+    assert np.array_equal(np.sort(son[son >= 0]), np.arange(1, nn))
+    assert np.array_equal(np.sort(-son[son < 0] - 1), np.arange(w.nleaf))
+    # leaf boxes hold their particles (grid positions, FP32 boxes)
+    lo = t["leaf_center"] - 0.5 * t["leaf_width"]
+    hi = t["leaf_center"] + 0.5 * t["leaf_width"]
+    x = (m["posi"] * m["int2pos"]).astype(np.float64)
+    leaf_of = np.repeat(np.arange(w.nleaf), w.leaf_np)
+    tol = 1e-4 * np.abs(x).max()
+    assert np.all(x >= lo[leaf_of] - tol) and np.all(x <= hi[leaf_of] + tol)
+    # tree alone, then the walk: the same list as the one-step build
+    if f is not None:
+        args = (f["posi"], f["tag"], float(f["int2pos"]), int(f["maxleaf"]), np.float32(f["open_angle"]),
+                np.float32(f["cutoff_radius"]), f["bdl"], f["bdr"])
+        a = W.build_lists(*args, walk=False)
+        assert a["task_t"].size == 0 and np.array_equal(a["perm"], f["perm"])
