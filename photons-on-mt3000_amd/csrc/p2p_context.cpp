@@ -128,14 +128,15 @@ struct pp2p_ctx {
         DevBuf<int2> f;
         DevBuf<unsigned char> code;
         DevBuf<unsigned> off;
-        DevBuf<long long> x;
+        DevBuf<int> x;
         long long n = 0;
     };
     DevBuf<int2> wson;
     DevBuf<float4> wnc, wnw, wlc, wlw;
     DevBuf<int> wact;
     std::vector<WalkRound> wr;
-    DevBuf<long long> wcnt, wpos_a, wpos_b;
+    DevBuf<int> wcnt, wpos_a, wpos_b;
+    DevBuf<unsigned long long> wemit;
 
     float last_kernel_ms = 0.f, last_prepare_ms = 0.f;
     bool kernel_timed = false, prepare_timed = false;
@@ -218,7 +219,7 @@ void pp2p_destroy(pp2p_ctx *c)
     for (auto &r : c->wr) {
         r.f.release(); r.code.release(); r.off.release(); r.x.release();
     }
-    c->wcnt.release(); c->wpos_a.release(); c->wpos_b.release();
+    c->wcnt.release(); c->wpos_a.release(); c->wpos_b.release(); c->wemit.release();
     c->idx.release(); c->idx_sorted.release(); c->counter.release(); c->temp.release();
     c->pair_count.release();
     if (c->ev_k0) (void)hipEventDestroy(c->ev_k0);
@@ -517,6 +518,8 @@ int pp2p_walk_tasks(pp2p_ctx *c, int nnode, const int *node_son, const float *no
     const int2 root = nnode > 0 ? make_int2(0, 0) : make_int2(-1, -1);
     HIPCHK(hipMemcpyAsync(wr[0].f.p, &root, sizeof(int2), hipMemcpyHostToDevice, c->stream));
     wr[0].n = 1;
+    // per round: frames that emit a task (the list length is their sum)
+    HIPCHK(c->wemit.reserve(2 * (size_t)nnode + 4));
     int last = 0;
     for (int r = 0;; ++r) {
         if (r > 2 * nnode + 2)
@@ -526,13 +529,13 @@ int pp2p_walk_tasks(pp2p_ctx *c, int nnode, const int *node_son, const float *no
         if (n > (1ll << 29))
             return fail(PP2P_ERR_ARG, "tree walk round larger than 2^29 frames");
         size_t tb = 0;
-        HIPCHK(walk_code(t, wr[r].f.p, n, nullptr, nullptr, nullptr, &tb, c->stream));
+        HIPCHK(walk_code(t, wr[r].f.p, n, nullptr, nullptr, nullptr, nullptr, &tb, c->stream));
         if ((st = ensure_temp(c, tb)))
             return st;
         tb = c->temp.cap;
         HIPCHK(wr[r].code.reserve(n));
         HIPCHK(wr[r].off.reserve(n + 1));
-        HIPCHK(walk_code(t, wr[r].f.p, n, wr[r].code.p, wr[r].off.p, c->temp.p, &tb, c->stream));
+        HIPCHK(walk_code(t, wr[r].f.p, n, wr[r].code.p, wr[r].off.p, c->wemit.p + r, c->temp.p, &tb, c->stream));
         unsigned nn = 0;
         HIPCHK(hipMemcpyAsync(&nn, wr[r].off.p + n, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
@@ -546,6 +549,15 @@ int pp2p_walk_tasks(pp2p_ctx *c, int nnode, const int *node_son, const float *no
         HIPCHK(walk_children(t, wr[r].f.p, n, wr[r].off.p, wr[r + 1].f.p, c->stream));
         wr[r + 1].n = nn;
     }
+    std::vector<unsigned long long> em(last + 1);
+    HIPCHK(hipMemcpyAsync(em.data(), c->wemit.p, sizeof(unsigned long long) * (last + 1), hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    unsigned long long emitted = 0;
+    for (unsigned long long v : em)
+        emitted += v;
+    if (emitted > (1ull << 31) - 1)
+        return fail(PP2P_ERR_ARG, "more than 2^31-1 tasks in one list");
     // backward: tasks at and below each frame, last round first (every
     // buffer the passes share is sized before the first launch)
     long long nmax = 1;
@@ -563,23 +575,19 @@ int pp2p_walk_tasks(pp2p_ctx *c, int nnode, const int *node_son, const float *no
     HIPCHK(c->wpos_a.reserve(nmax));
     HIPCHK(c->wpos_b.reserve(nmax));
     for (int r = last; r >= 0; --r) {
-        const long long *xn = r < last ? wr[r + 1].x.p : nullptr;
+        const int *xn = r < last ? wr[r + 1].x.p : nullptr;
         size_t tb = c->temp.cap;
         HIPCHK(walk_below(wr[r].code.p, wr[r].off.p, xn, wr[r].n, c->wcnt.p, wr[r].x.p, c->temp.p, &tb, c->stream));
     }
-    long long total = 0;
-    HIPCHK(hipMemcpyAsync(&total, wr[0].x.p + 1, sizeof(long long), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    if (total > (1ll << 31) - 1)
-        return fail(PP2P_ERR_ARG, "more than 2^31-1 tasks in one list");
+    const long long total = (long long)emitted;
     // positions: first task of each frame, round by round; emitting frames write
     HIPCHK(c->task_t.reserve(std::max(total, 1ll)));
     HIPCHK(c->task_s.reserve(std::max(total, 1ll)));
-    const long long zero = 0;
-    HIPCHK(hipMemcpyAsync(c->wpos_a.p, &zero, sizeof(long long), hipMemcpyHostToDevice, c->stream));
-    DevBuf<long long> *pc = &c->wpos_a, *pn = &c->wpos_b;
+    const int zero = 0;
+    HIPCHK(hipMemcpyAsync(c->wpos_a.p, &zero, sizeof(int), hipMemcpyHostToDevice, c->stream));
+    DevBuf<int> *pc = &c->wpos_a, *pn = &c->wpos_b;
     for (int r = 0; r <= last; ++r) {
-        const long long *xn = r < last ? wr[r + 1].x.p : nullptr;
+        const int *xn = r < last ? wr[r + 1].x.p : nullptr;
         HIPCHK(walk_emit(wr[r].f.p, wr[r].code.p, wr[r].off.p, xn, wr[r].n, pc->p, r < last ? pn->p : nullptr,
                          c->task_t.p, c->task_s.p, c->stream));
         std::swap(pc, pn);

@@ -17,20 +17,22 @@ struct WalkTree {
 };
 
 // Forward pass, round of n frames f: code[i] = children (0, 2, 4) | emit << 3,
-// off[0..n] = exclusive scan of the children (off[n] = next round's size).
+// off[0..n] = exclusive scan of the children (off[n] = next round's size),
+// *emitted = frames of the round that emit a task.
 // rocPRIM convention: temp == nullptr only queries *temp_bytes.
-hipError_t walk_code(const WalkTree &t, const int2 *f, long long n, unsigned char *code, unsigned *off, void *temp,
-                     size_t *temp_bytes, hipStream_t st);
+hipError_t walk_code(const WalkTree &t, const int2 *f, long long n, unsigned char *code, unsigned *off,
+                     unsigned long long *emitted, void *temp, size_t *temp_bytes, hipStream_t st);
 // The children of round f at their offsets in next.
 hipError_t walk_children(const WalkTree &t, const int2 *f, long long n, const unsigned *off, int2 *next,
                          hipStream_t st);
 // Backward pass: cnt[i] = tasks at and below frame i (xn: the next round's x,
-// null for the last round), x[0..n] = exclusive scan of cnt.
-hipError_t walk_below(const unsigned char *code, const unsigned *off, const long long *xn, long long n,
-                      long long *cnt, long long *x, void *temp, size_t *temp_bytes, hipStream_t st);
+// null for the last round), x[0..n] = exclusive scan of cnt.  32-bit: the
+// caller checks that the list length (the emitted count) fits.
+hipError_t walk_below(const unsigned char *code, const unsigned *off, const int *xn, long long n, int *cnt, int *x,
+                      void *temp, size_t *temp_bytes, hipStream_t st);
 // Position pass: pos[i] = first task position of frame i; fills pos_next for
 // the next round and writes the tasks of the emitting frames.
-hipError_t walk_emit(const int2 *f, const unsigned char *code, const unsigned *off, const long long *xn, long long n,
-                     const long long *pos, long long *pos_next, int *task_t, int *task_s, hipStream_t st);
+hipError_t walk_emit(const int2 *f, const unsigned char *code, const unsigned *off, const int *xn, long long n,
+                     const int *pos, int *pos_next, int *task_t, int *task_s, hipStream_t st);
 
 }  // namespace pp2p

@@ -128,6 +128,9 @@ __device__ __forceinline__ int expand_frame(const WalkTree &t, int2 f, int2 *ch,
 struct KidsOf {
     __host__ __device__ unsigned operator()(unsigned char c) const { return c & 7u; }
 };
+struct EmitOf {
+    __host__ __device__ unsigned long long operator()(unsigned char c) const { return c >> 3; }
+};
 
 inline unsigned grid_of(long long n) { return (unsigned)((n + kWalkBlock - 1) / kWalkBlock); }
 
@@ -168,14 +171,14 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_children(WalkTree t, const 
 // backward: tasks at and below each frame
 __global__ __launch_bounds__(kWalkBlock) void k_walk_below(const unsigned char *__restrict__ code,
                                                          const unsigned *__restrict__ off,
-                                                         const long long *__restrict__ xn, long long n,
-                                                         long long *__restrict__ cnt)
+                                                         const int *__restrict__ xn, long long n,
+                                                         int *__restrict__ cnt)
 {
     const long long i = (long long)blockIdx.x * kWalkBlock + threadIdx.x;
     if (i >= n)
         return;
     const unsigned c = code[i];
-    long long v = c >> 3;
+    int v = c >> 3;
     if (c & 7u) {
         const unsigned o = off[i];
         v += xn[o + (c & 7u)] - xn[o];
@@ -183,7 +186,7 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_below(const unsigned char *
     cnt[i] = v;
 }
 
-__global__ void k_walk_x_end(long long *__restrict__ x, const long long *__restrict__ cnt, long long n)
+__global__ void k_walk_x_end(int *__restrict__ x, const int *__restrict__ cnt, long long n)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0)
         x[n] = x[n - 1] + cnt[n - 1];
@@ -193,16 +196,16 @@ __global__ void k_walk_x_end(long long *__restrict__ x, const long long *__restr
 __global__ __launch_bounds__(kWalkBlock) void k_walk_emit(const int2 *__restrict__ f,
                                                         const unsigned char *__restrict__ code,
                                                         const unsigned *__restrict__ off,
-                                                        const long long *__restrict__ xn, long long n,
-                                                        const long long *__restrict__ pos,
-                                                        long long *__restrict__ pos_next, int *__restrict__ task_t,
+                                                        const int *__restrict__ xn, long long n,
+                                                        const int *__restrict__ pos,
+                                                        int *__restrict__ pos_next, int *__restrict__ task_t,
                                                         int *__restrict__ task_s)
 {
     const long long i = (long long)blockIdx.x * kWalkBlock + threadIdx.x;
     if (i >= n)
         return;
     const unsigned c = code[i];
-    const long long p = pos[i];
+    const int p = pos[i];
     if (c >> 3) {
         const int2 fr = f[i];
         task_t[p] = -fr.x - 1;
@@ -212,7 +215,7 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_emit(const int2 *__restrict
     const unsigned k = c & 7u;
     if (k) {
         const unsigned o = off[i];
-        const long long x0 = xn[o];
+        const int x0 = xn[o];
         for (unsigned j = 0; j < k; ++j)
             pos_next[o + j] = p + (xn[o + j] - x0);
     }
@@ -220,14 +223,24 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_emit(const int2 *__restrict
 
 }  // namespace
 
-hipError_t walk_code(const WalkTree &t, const int2 *f, long long n, unsigned char *code, unsigned *off, void *temp,
-                     size_t *temp_bytes, hipStream_t st)
+hipError_t walk_code(const WalkTree &t, const int2 *f, long long n, unsigned char *code, unsigned *off,
+                     unsigned long long *emitted, void *temp, size_t *temp_bytes, hipStream_t st)
 {
     auto kids = rocprim::make_transform_iterator(code, KidsOf());
-    if (!temp)
-        return rocprim::exclusive_scan(nullptr, *temp_bytes, kids, off, 0u, (size_t)n, rocprim::plus<unsigned>(), st);
+    auto emits = rocprim::make_transform_iterator(code, EmitOf());
+    if (!temp) {
+        size_t a = 0, b = 0;
+        hipError_t e = rocprim::exclusive_scan(nullptr, a, kids, off, 0u, (size_t)n, rocprim::plus<unsigned>(), st);
+        if (e == hipSuccess)
+            e = rocprim::reduce(nullptr, b, emits, emitted, 0ull, (size_t)n, rocprim::plus<unsigned long long>(), st);
+        *temp_bytes = a > b ? a : b;
+        return e;
+    }
     k_walk_code<<<grid_of(n), kWalkBlock, 0, st>>>(t, f, n, code);
     hipError_t e = rocprim::exclusive_scan(temp, *temp_bytes, kids, off, 0u, (size_t)n, rocprim::plus<unsigned>(), st);
+    if (e == hipSuccess)
+        e = rocprim::reduce(temp, *temp_bytes, emits, emitted, 0ull, (size_t)n, rocprim::plus<unsigned long long>(),
+                            st);
     if (e != hipSuccess)
         return e;
     k_walk_off_end<<<1, 64, 0, st>>>(off, code, n);
@@ -241,21 +254,21 @@ hipError_t walk_children(const WalkTree &t, const int2 *f, long long n, const un
     return hipGetLastError();
 }
 
-hipError_t walk_below(const unsigned char *code, const unsigned *off, const long long *xn, long long n,
-                      long long *cnt, long long *x, void *temp, size_t *temp_bytes, hipStream_t st)
+hipError_t walk_below(const unsigned char *code, const unsigned *off, const int *xn, long long n, int *cnt, int *x,
+                      void *temp, size_t *temp_bytes, hipStream_t st)
 {
     if (!temp)
-        return rocprim::exclusive_scan(nullptr, *temp_bytes, cnt, x, 0ll, (size_t)n, rocprim::plus<long long>(), st);
+        return rocprim::exclusive_scan(nullptr, *temp_bytes, cnt, x, 0, (size_t)n, rocprim::plus<int>(), st);
     k_walk_below<<<grid_of(n), kWalkBlock, 0, st>>>(code, off, xn, n, cnt);
-    hipError_t e = rocprim::exclusive_scan(temp, *temp_bytes, cnt, x, 0ll, (size_t)n, rocprim::plus<long long>(), st);
+    hipError_t e = rocprim::exclusive_scan(temp, *temp_bytes, cnt, x, 0, (size_t)n, rocprim::plus<int>(), st);
     if (e != hipSuccess)
         return e;
     k_walk_x_end<<<1, 64, 0, st>>>(x, cnt, n);
     return hipGetLastError();
 }
 
-hipError_t walk_emit(const int2 *f, const unsigned char *code, const unsigned *off, const long long *xn, long long n,
-                     const long long *pos, long long *pos_next, int *task_t, int *task_s, hipStream_t st)
+hipError_t walk_emit(const int2 *f, const unsigned char *code, const unsigned *off, const int *xn, long long n,
+                     const int *pos, int *pos_next, int *task_t, int *task_s, hipStream_t st)
 {
     k_walk_emit<<<grid_of(n), kWalkBlock, 0, st>>>(f, code, off, xn, n, pos, pos_next, task_t, task_s);
     return hipGetLastError();
