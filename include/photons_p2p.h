@@ -157,6 +157,28 @@ int pp2p_add_tasks(pp2p_ctx *ctx, long long ntask, const int *task_t, const int 
 int pp2p_walk_tasks(pp2p_ctx *ctx, int nnode, const int *node_son, const float *node_center,
                     const float *node_width, const float *leaf_center, const float *leaf_width,
                     const int *leaf_active, float open_angle, float cutoff_radius, long long *ntask);
+/*
+ * The kd-tree on the device (build_localtree, src/fmm.c:168-327, and the
+ * target marking of src/fmm.c:1226-1236), node for node, leaf for leaf and in
+ * the particle order the reference's sequential build gives: mean splits
+ * cycling x, y, z from the root box bdl..bdr, leaves of at most maxleaf
+ * particles.  posi: n x 3 int32 grid positions (position = int2pos * posi);
+ * tag: 1 local particle, 0 ghost copy.  Sets the context's particles in tree
+ * order and its leaves (as pp2p_set_particles_i32 would with the tree's
+ * permutation) and keeps the tree for pp2p_walk_tree.  *nleaf / *nnode
+ * (nullable) receive the sizes.  Synchronous.
+ */
+int pp2p_build_tree(pp2p_ctx *ctx, long long n, const int *posi, const int *tag, double int2pos, int maxleaf,
+                    const float *bdl, const float *bdr, int *nleaf, int *nnode);
+/* pp2p_walk_tasks over the tree of the last pp2p_build_tree. */
+int pp2p_walk_tree(pp2p_ctx *ctx, float open_angle, float cutoff_radius, long long *ntask);
+/* Copies out the tree of the last pp2p_build_tree (each pointer nullable):
+ * perm[k] = input index of the particle at tree position k (npart); leaf
+ * ranges, target marks (0 / -1), centres and widths (nleaf x 3); node sons
+ * (nnode x 2, >= 0 node, < 0 leaf -(id + 1)), centres and widths (nnode x 3).
+ * Synchronous. */
+int pp2p_get_tree(pp2p_ctx *ctx, int *perm, int *leaf_ip, int *leaf_np, int *leaf_active, float *leaf_center,
+                  float *leaf_width, int *node_son, float *node_center, float *node_width);
 /* Copies the context's current list out (each pointer nullable); *ntask
  * (nullable) receives its length.  Synchronous. */
 int pp2p_get_tasks(pp2p_ctx *ctx, long long *ntask, int *task_t, int *task_s);

@@ -14,6 +14,7 @@
 #include "forcelaw.h"
 #include "p2p_build.h"
 #include "p2p_launch.h"
+#include "p2p_tree.h"
 #include "p2p_walk.h"
 
 using namespace pp2p;
@@ -137,6 +138,23 @@ struct pp2p_ctx {
     std::vector<WalkRound> wr;
     DevBuf<int> wcnt, wpos_a, wpos_b;
     DevBuf<unsigned long long> wemit;
+    int walk_nnode = -1;        // nodes of the tree in the walk arrays (-1: none)
+
+    // device kd-tree (pp2p_build_tree): per level its segments and sons, and
+    // the per-particle arrays of the build; the finished tree goes to the
+    // walk arrays above, its leaf ranges to rng
+    struct TreeLevel {
+        DevBuf<int2> seg, kid, leaf, below, idb;
+        DevBuf<float> split;
+        DevBuf<double> mean;
+        DevBuf<int> np0, kk;
+        DevBuf<float4> lo, hi;
+        int nseg = 0, nleaf = 0;
+    };
+    std::vector<TreeLevel> tl;
+    DevBuf<int> tx0, tx1, tx2, tidx, tsegof, tsmall, tsx, ttbl, ttag, tposi;
+    DevBuf<int> tnodef, tleaff, tnoff, tloff;
+    bool tree_ready = false;    // walk arrays, tidx and rng hold the last pp2p_build_tree
 
     float last_kernel_ms = 0.f, last_prepare_ms = 0.f;
     bool kernel_timed = false, prepare_timed = false;
@@ -220,6 +238,13 @@ void pp2p_destroy(pp2p_ctx *c)
         r.f.release(); r.code.release(); r.off.release(); r.x.release();
     }
     c->wcnt.release(); c->wpos_a.release(); c->wpos_b.release(); c->wemit.release();
+    for (auto &l : c->tl) {
+        l.seg.release(); l.kid.release(); l.leaf.release(); l.below.release(); l.idb.release(); l.split.release();
+        l.mean.release(); l.np0.release(); l.kk.release(); l.lo.release(); l.hi.release();
+    }
+    c->tx0.release(); c->tx1.release(); c->tx2.release(); c->tidx.release(); c->tsegof.release();
+    c->tsmall.release(); c->tsx.release(); c->ttbl.release(); c->ttag.release(); c->tposi.release();
+    c->tnodef.release(); c->tleaff.release(); c->tnoff.release(); c->tloff.release();
     c->idx.release(); c->idx_sorted.release(); c->counter.release(); c->temp.release();
     c->pair_count.release();
     if (c->ev_k0) (void)hipEventDestroy(c->ev_k0);
@@ -324,6 +349,7 @@ static int finish_particles(pp2p_ctx *c, long long npart, int nleaf)
     c->npart = npart;
     c->nleaf = nleaf;
     c->have_particles = true;
+    c->tree_ready = false;
     c->has_act = false;
     c->ntask = 0;
     c->csr_ready = c->items_ready = false;
@@ -472,40 +498,12 @@ static int upload_boxes(pp2p_ctx *c, DevBuf<float4> &dst, int n, const float *xy
     return PP2P_OK;
 }
 
-int pp2p_walk_tasks(pp2p_ctx *c, int nnode, const int *node_son, const float *node_center, const float *node_width,
-                    const float *leaf_center, const float *leaf_width, const int *leaf_active, float open_angle,
-                    float cutoff_radius, long long *ntask)
+// The walk over the tree in the walk arrays (c->walk_nnode nodes, c->nleaf
+// leaves): replaces the context's list.
+static int run_walk(pp2p_ctx *c, float open_angle, float cutoff_radius, long long *ntask)
 {
-    int st = check_ctx(c);
-    if (st)
-        return st;
-    if (!c->have_particles)
-        return fail(PP2P_ERR_STATE, "set_particles before walk_tasks");
-    const int nleaf = c->nleaf;
-    if (nnode < 0 || (nnode > 0 && (!node_son || !node_center || !node_width)) || !leaf_center || !leaf_width ||
-        !leaf_active)
-        return fail(PP2P_ERR_ARG, "invalid tree arrays");
-    if (nnode == 0 && nleaf != 1)
-        return fail(PP2P_ERR_ARG, "a tree without nodes holds exactly one leaf");
-    if (!(open_angle >= 0.f) || !(cutoff_radius >= 0.f) || std::isinf(cutoff_radius))
-        return fail(PP2P_ERR_ARG, "invalid open_angle or cutoff_radius");
-    // sons are created after their parent (ids increase down the tree), so
-    // every round moves at least one cell of each opened frame one level down
-    // and the walk ends within 2 * nnode + 2 rounds
-    for (int k = 0; k < nnode; ++k)
-        for (int h = 0; h < 2; ++h) {
-            const int s = node_son[2 * k + h];
-            if (s >= 0 ? (s <= k || s >= nnode) : (-(long long)s - 1 >= nleaf))
-                return fail(PP2P_ERR_ARG, "node " + std::to_string(k) + " has an invalid son");
-        }
-    HIPCHK(c->wson.reserve(std::max(nnode, 1)));
-    if (nnode)
-        HIPCHK(hipMemcpyAsync(c->wson.p, node_son, sizeof(int2) * nnode, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c->wact.reserve(nleaf));
-    HIPCHK(hipMemcpyAsync(c->wact.p, leaf_active, sizeof(int) * nleaf, hipMemcpyHostToDevice, c->stream));
-    if ((st = upload_boxes(c, c->wnc, nnode, node_center)) || (st = upload_boxes(c, c->wnw, nnode, node_width)) ||
-        (st = upload_boxes(c, c->wlc, nleaf, leaf_center)) || (st = upload_boxes(c, c->wlw, nleaf, leaf_width)))
-        return st;
+    const int nnode = c->walk_nnode;
+    int st = PP2P_OK;
     WalkTree t{c->wson.p, c->wnc.p, c->wnw.p, c->wlc.p, c->wlw.p, c->wact.p, open_angle, cutoff_radius};
 
     c->ntask = 0;
@@ -596,6 +594,229 @@ int pp2p_walk_tasks(pp2p_ctx *c, int nnode, const int *node_son, const float *no
     c->ntask = total;
     if (ntask)
         *ntask = c->ntask;
+    return PP2P_OK;
+}
+
+int pp2p_walk_tasks(pp2p_ctx *c, int nnode, const int *node_son, const float *node_center, const float *node_width,
+                    const float *leaf_center, const float *leaf_width, const int *leaf_active, float open_angle,
+                    float cutoff_radius, long long *ntask)
+{
+    int st = check_ctx(c);
+    if (st)
+        return st;
+    if (!c->have_particles)
+        return fail(PP2P_ERR_STATE, "set_particles before walk_tasks");
+    const int nleaf = c->nleaf;
+    if (nnode < 0 || (nnode > 0 && (!node_son || !node_center || !node_width)) || !leaf_center || !leaf_width ||
+        !leaf_active)
+        return fail(PP2P_ERR_ARG, "invalid tree arrays");
+    if (nnode == 0 && nleaf != 1)
+        return fail(PP2P_ERR_ARG, "a tree without nodes holds exactly one leaf");
+    if (!(open_angle >= 0.f) || !(cutoff_radius >= 0.f) || std::isinf(cutoff_radius))
+        return fail(PP2P_ERR_ARG, "invalid open_angle or cutoff_radius");
+    // sons are created after their parent (ids increase down the tree), so
+    // every round moves at least one cell of each opened frame one level down
+    // and the walk ends within 2 * nnode + 2 rounds
+    for (int k = 0; k < nnode; ++k)
+        for (int h = 0; h < 2; ++h) {
+            const int s = node_son[2 * k + h];
+            if (s >= 0 ? (s <= k || s >= nnode) : (-(long long)s - 1 >= nleaf))
+                return fail(PP2P_ERR_ARG, "node " + std::to_string(k) + " has an invalid son");
+        }
+    HIPCHK(c->wson.reserve(std::max(nnode, 1)));
+    if (nnode)
+        HIPCHK(hipMemcpyAsync(c->wson.p, node_son, sizeof(int2) * nnode, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c->wact.reserve(nleaf));
+    HIPCHK(hipMemcpyAsync(c->wact.p, leaf_active, sizeof(int) * nleaf, hipMemcpyHostToDevice, c->stream));
+    if ((st = upload_boxes(c, c->wnc, nnode, node_center)) || (st = upload_boxes(c, c->wnw, nnode, node_width)) ||
+        (st = upload_boxes(c, c->wlc, nleaf, leaf_center)) || (st = upload_boxes(c, c->wlw, nleaf, leaf_width)))
+        return st;
+    c->walk_nnode = nnode;
+    c->tree_ready = false;
+    return run_walk(c, open_angle, cutoff_radius, ntask);
+}
+
+int pp2p_build_tree(pp2p_ctx *c, long long n, const int *posi, const int *tag, double int2pos, int maxleaf,
+                    const float *bdl, const float *bdr, int *nleaf_out, int *nnode_out)
+{
+    int st = check_ctx(c);
+    if (st)
+        return st;
+    if (!c->have_params)
+        return fail(PP2P_ERR_STATE, "set_params before build_tree");
+    if (n < 1 || n > (1ll << 31) - 2 || !posi || !tag || !bdl || !bdr)
+        return fail(PP2P_ERR_ARG, "invalid particle arrays");
+    if (maxleaf < 1 || !(int2pos > 0.0) || std::isinf(int2pos))
+        return fail(PP2P_ERR_ARG, "invalid maxleaf or int2pos");
+    const hipStream_t sm = c->stream;
+    HIPCHK(c->tposi.reserve(3 * n));
+    HIPCHK(c->ttag.reserve(n));
+    for (DevBuf<int> *b : {&c->tx0, &c->tx1, &c->tx2, &c->tidx, &c->tsegof, &c->ttbl})
+        HIPCHK(b->reserve(n));
+    HIPCHK(c->tsmall.reserve(n + 1));
+    HIPCHK(c->tsx.reserve(n + 1));
+    HIPCHK(hipMemcpyAsync(c->tposi.p, posi, sizeof(int) * 3 * n, hipMemcpyHostToDevice, sm));
+    HIPCHK(hipMemcpyAsync(c->ttag.p, tag, sizeof(int) * n, hipMemcpyHostToDevice, sm));
+    int *const xs[3] = {c->tx0.p, c->tx1.p, c->tx2.p};
+    const TreeScratch w{{c->tx0.p, c->tx1.p, c->tx2.p}, c->tidx.p, c->tsegof.p, c->tsmall.p, c->tsx.p, c->ttbl.p};
+    HIPCHK(tree_init(c->tposi.p, n, xs, c->tidx.p, c->tsegof.p, sm));
+    auto &tl = c->tl;
+    if (tl.empty())
+        tl.resize(1);
+    HIPCHK(tl[0].seg.reserve(1));
+    const int2 root = make_int2(0, (int)n);
+    HIPCHK(hipMemcpyAsync(tl[0].seg.p, &root, sizeof(int2), hipMemcpyHostToDevice, sm));
+    tl[0].nseg = 1;
+    // levels: split every segment, then its sons become the next level or leaves
+    int last = 0;
+    for (int l = 0;; ++l) {
+        if (l >= 1024)
+            return fail(PP2P_ERR_ARG, "tree deeper than 1024 levels (many coincident particles?)");
+        HIPCHK(hipStreamSynchronize(sm));      // nothing in flight while buffers grow
+        auto &L = tl[l];
+        const int ns = L.nseg;
+        HIPCHK(L.mean.reserve(ns));
+        HIPCHK(L.split.reserve(ns));
+        HIPCHK(L.np0.reserve(ns));
+        HIPCHK(L.kk.reserve(ns));
+        HIPCHK(L.kid.reserve(ns));
+        for (DevBuf<int> *b : {&c->tnodef, &c->tleaff, &c->tnoff, &c->tloff})
+            HIPCHK(b->reserve(2 * (size_t)ns + 1));
+        size_t t1 = 0, t2 = 0;
+        HIPCHK(tree_split(w, L.seg.p, ns, l % 3, int2pos, n, nullptr, nullptr, nullptr, nullptr, nullptr, &t1, sm));
+        HIPCHK(tree_children(L.seg.p, ns, nullptr, maxleaf, nullptr, nullptr, nullptr, nullptr, nullptr, &t2, sm));
+        if ((st = ensure_temp(c, std::max(t1, t2))))
+            return st;
+        t1 = t2 = c->temp.cap;
+        HIPCHK(tree_split(w, L.seg.p, ns, l % 3, int2pos, n, L.mean.p, L.split.p, L.np0.p, L.kk.p, c->temp.p, &t1,
+                          sm));
+        HIPCHK(tree_children(L.seg.p, ns, L.np0.p, maxleaf, c->tnodef.p, c->tleaff.p, c->tnoff.p, c->tloff.p,
+                             c->temp.p, &t2, sm));
+        int tot[2] = {0, 0};
+        HIPCHK(hipMemcpyAsync(&tot[0], c->tnoff.p + 2 * ns, sizeof(int), hipMemcpyDeviceToHost, sm));
+        HIPCHK(hipMemcpyAsync(&tot[1], c->tloff.p + 2 * ns, sizeof(int), hipMemcpyDeviceToHost, sm));
+        HIPCHK(hipStreamSynchronize(sm));
+        if ((int)tl.size() < l + 2)
+            tl.resize(l + 2);
+        auto &N = tl[l + 1];
+        HIPCHK(N.seg.reserve(std::max(tot[0], 1)));
+        HIPCHK(tl[l].leaf.reserve(std::max(tot[1], 1)));
+        N.nseg = tot[0];
+        tl[l].nleaf = tot[1];
+        HIPCHK(tree_next(w, tl[l].seg.p, ns, tl[l].np0.p, c->tnodef.p, c->tnoff.p, c->tloff.p, N.seg.p, tl[l].leaf.p,
+                         tl[l].kid.p, n, sm));
+        if (tot[0] == 0) {
+            last = l;
+            break;
+        }
+    }
+    HIPCHK(hipStreamSynchronize(sm));
+    // subtree sizes, deepest level first
+    for (int l = last; l >= 0; --l) {
+        HIPCHK(tl[l].below.reserve(tl[l].nseg));
+        HIPCHK(tl[l].idb.reserve(tl[l].nseg));
+        HIPCHK(tl[l].lo.reserve(tl[l].nseg));
+        HIPCHK(tl[l].hi.reserve(tl[l].nseg));
+    }
+    for (int l = last; l >= 0; --l)
+        HIPCHK(tree_below(tl[l].kid.p, tl[l].nseg, l < last ? tl[l + 1].below.p : nullptr, tl[l].below.p, sm));
+    int2 counts;
+    HIPCHK(hipMemcpyAsync(&counts, tl[0].below.p, sizeof(int2), hipMemcpyDeviceToHost, sm));
+    HIPCHK(hipStreamSynchronize(sm));
+    const int nnode = counts.x, nleaf = counts.y;
+    // ids and boxes, root first
+    HIPCHK(c->wson.reserve(nnode));
+    HIPCHK(c->wnc.reserve(nnode));
+    HIPCHK(c->wnw.reserve(nnode));
+    HIPCHK(c->wlc.reserve(nleaf));
+    HIPCHK(c->wlw.reserve(nleaf));
+    HIPCHK(c->wact.reserve(nleaf));
+    HIPCHK(c->rng.reserve(nleaf));
+    HIPCHK(c->x3.reserve(3 * n));
+    const int2 idb0 = make_int2(0, 0);
+    const float4 lo0 = make_float4(bdl[0], bdl[1], bdl[2], 0.f), hi0 = make_float4(bdr[0], bdr[1], bdr[2], 0.f);
+    HIPCHK(hipMemcpyAsync(tl[0].idb.p, &idb0, sizeof(int2), hipMemcpyHostToDevice, sm));
+    HIPCHK(hipMemcpyAsync(tl[0].lo.p, &lo0, sizeof(float4), hipMemcpyHostToDevice, sm));
+    HIPCHK(hipMemcpyAsync(tl[0].hi.p, &hi0, sizeof(float4), hipMemcpyHostToDevice, sm));
+    const TreeOut out{c->wson.p, c->wnc.p, c->wnw.p, c->wlc.p, c->wlw.p, c->rng.p, c->wact.p};
+    auto view = [&](int l) {
+        if (l > last)
+            return TreeLevelView{0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+        auto &L = tl[l];
+        return TreeLevelView{L.nseg, L.kid.p, L.leaf.p, L.split.p, L.below.p, L.idb.p, L.lo.p, L.hi.p};
+    };
+    for (int l = 0; l <= last; ++l)
+        HIPCHK(tree_place(view(l), view(l + 1), l % 3, out, sm));
+    HIPCHK(tree_finish(w, out, nleaf, c->ttag.p, int2pos, n, c->x3.p, sm));
+    if ((st = finish_particles(c, n, nleaf)))
+        return st;
+    c->walk_nnode = nnode;
+    c->tree_ready = true;
+    if (nleaf_out)
+        *nleaf_out = nleaf;
+    if (nnode_out)
+        *nnode_out = nnode;
+    return PP2P_OK;
+}
+
+int pp2p_walk_tree(pp2p_ctx *c, float open_angle, float cutoff_radius, long long *ntask)
+{
+    int st = check_ctx(c);
+    if (st)
+        return st;
+    if (!c->tree_ready)
+        return fail(PP2P_ERR_STATE, "build_tree before walk_tree");
+    if (!(open_angle >= 0.f) || !(cutoff_radius >= 0.f) || std::isinf(cutoff_radius))
+        return fail(PP2P_ERR_ARG, "invalid open_angle or cutoff_radius");
+    return run_walk(c, open_angle, cutoff_radius, ntask);
+}
+
+static int copy_xyz(pp2p_ctx *c, const float4 *src, int n, float *dst)
+{
+    std::vector<float4> h((size_t)n);
+    if (n)
+        HIPCHK(hipMemcpyAsync(h.data(), src, sizeof(float4) * n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int k = 0; k < n; ++k) {
+        dst[3 * k] = h[k].x;
+        dst[3 * k + 1] = h[k].y;
+        dst[3 * k + 2] = h[k].z;
+    }
+    return PP2P_OK;
+}
+
+int pp2p_get_tree(pp2p_ctx *c, int *perm, int *leaf_ip, int *leaf_np, int *leaf_active, float *leaf_center,
+                  float *leaf_width, int *node_son, float *node_center, float *node_width)
+{
+    int st = check_ctx(c);
+    if (st)
+        return st;
+    if (!c->tree_ready)
+        return fail(PP2P_ERR_STATE, "no tree built on the device");
+    const int nleaf = c->nleaf, nnode = c->walk_nnode;
+    if (perm)
+        HIPCHK(hipMemcpyAsync(perm, c->tidx.p, sizeof(int) * c->npart, hipMemcpyDeviceToHost, c->stream));
+    if (leaf_active)
+        HIPCHK(hipMemcpyAsync(leaf_active, c->wact.p, sizeof(int) * nleaf, hipMemcpyDeviceToHost, c->stream));
+    if (node_son)
+        HIPCHK(hipMemcpyAsync(node_son, c->wson.p, sizeof(int2) * nnode, hipMemcpyDeviceToHost, c->stream));
+    if (leaf_ip || leaf_np) {
+        std::vector<int2> r((size_t)nleaf);
+        HIPCHK(hipMemcpyAsync(r.data(), c->rng.p, sizeof(int2) * nleaf, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        for (int l = 0; l < nleaf; ++l) {
+            if (leaf_ip)
+                leaf_ip[l] = r[l].x;
+            if (leaf_np)
+                leaf_np[l] = r[l].y;
+        }
+    }
+    if ((leaf_center && (st = copy_xyz(c, c->wlc.p, nleaf, leaf_center))) ||
+        (leaf_width && (st = copy_xyz(c, c->wlw.p, nleaf, leaf_width))) ||
+        (node_center && (st = copy_xyz(c, c->wnc.p, nnode, node_center))) ||
+        (node_width && (st = copy_xyz(c, c->wnw.p, nnode, node_width))))
+        return st;
+    HIPCHK(hipStreamSynchronize(c->stream));
     return PP2P_OK;
 }
 

@@ -59,7 +59,7 @@ CORE_SYMBOLS = (
     "pp2p_version", "pp2p_device_count", "pp2p_last_error", "pp2p_create", "pp2p_destroy",
     "pp2p_set_params", "pp2p_set_tuning", "pp2p_get_tuning", "pp2p_set_particles_f64",
     "pp2p_set_particles_i32", "pp2p_set_active", "pp2p_set_tasks", "pp2p_add_tasks",
-    "pp2p_walk_tasks", "pp2p_get_tasks",
+    "pp2p_walk_tasks", "pp2p_get_tasks", "pp2p_build_tree", "pp2p_walk_tree", "pp2p_get_tree",
     "pp2p_prepare", "pp2p_compute", "pp2p_zero_acc", "pp2p_run_graph", "pp2p_sync",
     "pp2p_get_acc_f64", "pp2p_get_acc_f32_soa", "pp2p_list_stats", "pp2p_last_kernel_ms",
     "pp2p_last_prepare_ms", "pp2p_count_inside", "pp2p_time_steps", "pp2p_law_eval",
@@ -97,6 +97,9 @@ def load(path: str | None = None) -> ctypes.CDLL:
         "pp2p_walk_tasks": (i32, [vp, i32, vp, vp, vp, vp, vp, vp, ctypes.c_float, ctypes.c_float,
                                   ctypes.POINTER(i64)]),
         "pp2p_get_tasks": (i32, [vp, ctypes.POINTER(i64), vp, vp]),
+        "pp2p_build_tree": (i32, [vp, i64, vp, vp, dbl, i32, vp, vp, ctypes.POINTER(i32), ctypes.POINTER(i32)]),
+        "pp2p_walk_tree": (i32, [vp, ctypes.c_float, ctypes.c_float, ctypes.POINTER(i64)]),
+        "pp2p_get_tree": (i32, [vp] + [vp] * 9),
         "pp2p_prepare": (i32, [vp]),
         "pp2p_compute": (i32, [vp, i32]),
         "pp2p_zero_acc": (i32, [vp]),
@@ -157,6 +160,7 @@ class Context:
             raise P2PError(st.value, self._lib.pp2p_last_error().decode())
         self.npart = 0
         self.nleaf = 0
+        self.nnode = 0
 
     def _chk(self, status: int):
         if status != OK:
@@ -260,6 +264,43 @@ class Context:
         self._chk(self._lib.pp2p_walk_tasks(self._h, son.shape[0], _ptr(son), _ptr(nc), _ptr(nw), _ptr(lc),
                                             _ptr(lw), _ptr(act), float(open_angle), float(cutoff_radius),
                                             ctypes.byref(n)))
+        return n.value
+
+    def build_tree(self, posi: np.ndarray, tag: np.ndarray, int2pos: float, maxleaf: int, bdl: np.ndarray,
+                   bdr: np.ndarray) -> tuple[int, int]:
+        """Builds the kd-tree of n x 3 int32 grid positions on the device and
+        makes the particles (in tree order) and leaves the context's.  Returns
+        (nleaf, nnode)."""
+        p = np.ascontiguousarray(posi, np.int32)
+        t = np.ascontiguousarray(tag, np.int32)
+        lo = np.ascontiguousarray(bdl, np.float32)
+        hi = np.ascontiguousarray(bdr, np.float32)
+        if p.ndim != 2 or p.shape[1] != 3 or t.shape != (p.shape[0],) or lo.shape != (3,) or hi.shape != (3,):
+            raise ValueError("posi must be n x 3, tag n, bdl and bdr 3")
+        nl, nn = ctypes.c_int(0), ctypes.c_int(0)
+        self._chk(self._lib.pp2p_build_tree(self._h, p.shape[0], _ptr(p), _ptr(t), float(int2pos), int(maxleaf),
+                                            _ptr(lo), _ptr(hi), ctypes.byref(nl), ctypes.byref(nn)))
+        self.npart, self.nleaf, self.nnode = p.shape[0], nl.value, nn.value
+        return nl.value, nn.value
+
+    def get_tree(self) -> dict:
+        """The tree of the last build_tree, in the layout build_lists returns."""
+        nl, nn = self.nleaf, self.nnode
+        out = {"perm": np.empty(self.npart, np.int32), "leaf_ip": np.empty(nl, np.int32),
+               "leaf_np": np.empty(nl, np.int32), "active": np.empty(nl, np.int32),
+               "leaf_center": np.empty((nl, 3), np.float32), "leaf_width": np.empty((nl, 3), np.float32),
+               "node_son": np.empty((nn, 2), np.int32), "node_center": np.empty((nn, 3), np.float32),
+               "node_width": np.empty((nn, 3), np.float32)}
+        self._chk(self._lib.pp2p_get_tree(self._h, *[_ptr(out[k]) for k in (
+            "perm", "leaf_ip", "leaf_np", "active", "leaf_center", "leaf_width", "node_son", "node_center",
+            "node_width")]))
+        return out
+
+    def walk_tree(self, open_angle: float, cutoff_radius: float) -> int:
+        """The device walk over the tree of the last build_tree; returns the
+        list length."""
+        n = ctypes.c_longlong(0)
+        self._chk(self._lib.pp2p_walk_tree(self._h, float(open_angle), float(cutoff_radius), ctypes.byref(n)))
         return n.value
 
     def get_tasks(self) -> tuple[np.ndarray, np.ndarray]:

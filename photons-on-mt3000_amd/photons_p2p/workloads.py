@@ -269,7 +269,7 @@ def tree_workload(name: str, pos: np.ndarray, box: float, mesh: int, maxleaf: in
                         bdl, bdr, nthreads, with_tree=True)
     perm = lists["perm"]
     m = dict(meta or {})
-    m.update({"int2pos": int2pos, "box_i": box_i, "posi": allp[perm], "tag": tag[perm],
+    m.update({"int2pos": int2pos, "box_i": box_i, "posi": allp[perm], "tag": tag[perm], "perm": perm,
               "local_index": local_index[perm], "nlocal": posi.shape[0], "maxleaf": maxleaf,
               "open_angle": open_angle, "cutoff_radius": cutoff_radius, "mesh": mesh,
               "bdl": bdl, "bdr": bdr, "active_leaf": lists["active"],
