@@ -193,7 +193,7 @@ def main():
                     help="also time the one-launch-per-target form the same way and report the ratio")
     ap.add_argument("--chunk-model", type=int, default=0, help="i-chunking: 0 default, -1 chunks of 64, >0 cost")
     ap.add_argument("--no-device-walk", action="store_true",
-                    help="skip building the list again by the device walk (tree workloads)")
+                    help="skip building the tree and the list again on the device (tree workloads)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -348,23 +348,49 @@ def main():
                 + (f", p99 {ent['rel_err']['p99']:.2e}" if "rel_err" in ent else ""))
     lists = None
     if "tree" in w.meta and args.active <= 0 and not args.no_device_walk:
-        # the same list built on the device from the host tree (best of 3 calls,
-        # tree upload included), checked pair for pair against the host walk
+        # the list built again on the device, checked against the host's:
+        # (1) the walk of the host tree (uploaded), (2) tree and walk both on the
+        # device from the input-order grid positions; best of 3 calls each
         m = w.meta
-        dts = []
-        for _ in range(3):
-            t1 = time.perf_counter()
-            nwalk = ctx.walk_tasks(m["tree"], m["active_leaf"], np.float32(m["open_angle"]),
-                                   np.float32(m["cutoff_radius"]))
-            dts.append(time.perf_counter() - t1)
-        tt, ts = ctx.get_tasks()
-        same = bool(np.array_equal(tt, w.task_t) and np.array_equal(ts, w.task_s))
-        del tt, ts
         ls = m["list_seconds"]
+
+        def best_of(fn, k=3):
+            dts = []
+            for _ in range(k):
+                t1 = time.perf_counter()
+                r = fn()
+                dts.append(time.perf_counter() - t1)
+            return r, 1e3 * min(dts)
+
+        nwalk, walk_ms = best_of(lambda: ctx.walk_tasks(m["tree"], m["active_leaf"], np.float32(m["open_angle"]),
+                                                        np.float32(m["cutoff_radius"])))
+        tt, ts = ctx.get_tasks()
+        same_walk = bool(np.array_equal(tt, w.task_t) and np.array_equal(ts, w.task_s))
+        perm = m["perm"]
+        posi_in = np.empty_like(m["posi"])
+        tag_in = np.empty_like(m["tag"])
+        posi_in[perm] = m["posi"]
+        tag_in[perm] = m["tag"]
+        # the upload alone for scale: the same grid positions through set_particles_i32
+        _, upload_ms = best_of(lambda: ctx.set_particles_i32(m["posi"], m["int2pos"], w.leaf_ip, w.leaf_np))
+        _, tree_ms = best_of(lambda: ctx.build_tree(posi_in, tag_in, m["int2pos"], m["maxleaf"], m["bdl"], m["bdr"]))
+        t = ctx.get_tree()
+        same_tree = bool(np.array_equal(t["perm"], perm) and np.array_equal(t["leaf_ip"], w.leaf_ip)
+                         and np.array_equal(t["leaf_np"], w.leaf_np) and np.array_equal(t["active"], m["active_leaf"])
+                         and all(np.array_equal(t[k], m["tree"][k]) for k in m["tree"]))
+        del t
+        _, walk2_ms = best_of(lambda: ctx.walk_tree(np.float32(m["open_angle"]), np.float32(m["cutoff_radius"])))
+        tt, ts = ctx.get_tasks()
+        same_walk2 = bool(np.array_equal(tt, w.task_t) and np.array_equal(ts, w.task_s))
+        del tt, ts
         lists = {"host_tree_s": ls["tree"], "host_walk_s": ls["walk"], "host_threads": threads,
-                 "device_walk_ms": 1e3 * min(dts), "ntask": int(nwalk), "identical_to_host": same}
-        log(f"[bench] lists: host tree {ls['tree']:.2f}s, host walk {ls['walk']:.2f}s on {threads} threads; "
-            f"device walk {1e3 * min(dts):.1f} ms ({nwalk} tasks, identical to the host walk: {same})")
+                 "device_walk_ms": walk_ms, "device_walk_identical": same_walk,
+                 "device_tree_ms": tree_ms, "device_tree_identical": same_tree, "set_particles_i32_ms": upload_ms,
+                 "device_tree_walk_ms": walk2_ms, "device_tree_walk_identical": same_walk2, "ntask": int(nwalk)}
+        log(f"[bench] lists: host tree {ls['tree']:.2f}s + walk {ls['walk']:.2f}s on {threads} threads; "
+            f"device walk of the host tree {walk_ms:.1f} ms (identical: {same_walk}); device tree {tree_ms:.1f} ms "
+            f"(identical: {same_tree}; the position upload alone {upload_ms:.1f} ms) + walk {walk2_ms:.1f} ms "
+            f"(identical: {same_walk2})")
     ctx.close()
 
     copy_cal = None

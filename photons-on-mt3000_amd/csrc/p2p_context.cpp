@@ -390,8 +390,22 @@ int pp2p_set_particles_i32(pp2p_ctx *c, long long npart, const void *posi, size_
     HIPCHK(c->posi.reserve(std::max<long long>(3 * npart, 1)));
     HIPCHK(c->x3.reserve(std::max<long long>(3 * npart, 1)));
     if (npart) {
-        HIPCHK(hipMemcpy2DAsync(c->posi.p, 12, posi, stride, 12, (size_t)npart, hipMemcpyHostToDevice, c->stream));
+        // one contiguous copy: records wider than the three coordinates are
+        // packed on the host first (as the reference repacks part[i].posi,
+        // src/gpu_entry.cu:85-105); a pitched copy of npart 12-byte rows from
+        // pageable memory is orders of magnitude slower
+        const void *src = posi;
+        std::vector<int> packed;
+        if (stride != 12) {
+            packed.resize(3 * (size_t)npart);
+            const unsigned char *b = static_cast<const unsigned char *>(posi);
+            for (long long i = 0; i < npart; ++i)
+                std::memcpy(&packed[3 * (size_t)i], b + (size_t)i * stride, 12);
+            src = packed.data();
+        }
+        HIPCHK(hipMemcpyAsync(c->posi.p, src, 12 * (size_t)npart, hipMemcpyHostToDevice, c->stream));
         HIPCHK(launch_i32_to_f64(c->posi.p, 3 * npart, int2pos, false, c->x3.p, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));      // before the packed copy goes out of scope
     }
     return finish_particles(c, npart, nleaf);
 }

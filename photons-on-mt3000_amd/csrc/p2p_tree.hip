@@ -57,29 +57,65 @@ __global__ __launch_bounds__(kTreeBlock) void k_tree_init(const int *__restrict_
 }
 
 // Split value of each segment (src/fmm.c:168-197): the sampled FP64 mean for
-// more than two particles, the midpoint of two, 0 for fewer.
+// more than two particles, the midpoint of two, 0 for fewer.  One wave per
+// segment: the lanes load 512 samples at a time into the wave's LDS slice,
+// and every lane adds them to its sum one after the other in sample order
+// (the same chain in every lane, each value a broadcast LDS read), so the sum
+// is the reference's sequential one.  Padding the last block with +0.0
+// leaves the sum unchanged: it is never -0.0 (it starts at +0.0 and an exact
+// cancellation rounds to +0.0).
+constexpr int kMeanBatch = 512;
+
 __global__ __launch_bounds__(kTreeBlock) void k_tree_mean(const int2 *__restrict__ seg, int nseg,
                                                         const int *__restrict__ x, double int2pos,
                                                         double *__restrict__ mean, float *__restrict__ split)
 {
-    const long long s = gid();
+    __shared__ double buf[kTreeBlock / 64][kMeanBatch];
+    const int s = blockIdx.x * (kTreeBlock / 64) + threadIdx.x / 64;
+    const int lane = threadIdx.x & 63;
     if (s >= nseg)
         return;
+    double *wb = buf[threadIdx.x / 64];
     const int a = seg[s].x, len = seg[s].y;
     double m = 0.0;
     float sp = 0.0f;
     if (len > 2) {
         const long long mul = len > 100000 ? len / 100000 : 1;
-        for (long long k = 0; k < len; k += mul)
-            m += int2pos * (double)x[a + k];
+        const long long ns = (len + mul - 1) / mul;
+        constexpr int kU = kMeanBatch / 64;
+        for (long long b = 0; b < ns; b += kMeanBatch) {
+            // indices past the end are clamped for the load and masked after
+            int r[kU];
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                const long long k = b + u * 64 + lane;
+                r[u] = x[a + (k < ns ? k : ns - 1) * mul];
+            }
+#pragma unroll
+            for (int u = 0; u < kU; ++u)
+                wb[u * 64 + lane] = b + u * 64 + lane < ns ? int2pos * (double)r[u] : 0.0;
+            __builtin_amdgcn_wave_barrier();
+            const int cnt = ns - b < kMeanBatch ? (int)(ns - b) : kMeanBatch;     // the same in every lane
+            int j = 0;
+            for (; j + 16 <= cnt; j += 16) {
+#pragma unroll
+                for (int q = 0; q < 16; ++q)
+                    m += wb[j + q];
+            }
+            for (; j < cnt; ++j)
+                m += wb[j];
+            __builtin_amdgcn_wave_barrier();
+        }
         m /= (double)len;
         m *= (double)mul;
         sp = (float)m;
     } else if (len == 2) {
         sp = (float)(0.5 * (int2pos * (double)(x[a] + x[a + 1])));
     }
-    mean[s] = m;
-    split[s] = sp;
+    if (lane == 0) {
+        mean[s] = m;
+        split[s] = sp;
+    }
 }
 
 __global__ __launch_bounds__(kTreeBlock) void k_tree_flags(const int *__restrict__ x, const int *__restrict__ segof,
@@ -376,7 +412,8 @@ hipError_t tree_split(const TreeScratch &w, const int2 *seg, int nseg, int dir, 
     if (!temp)
         return rocprim::exclusive_scan(nullptr, *temp_bytes, w.small, w.sx, 0, (size_t)n, rocprim::plus<int>(), st);
     const int *x = w.x[dir];
-    k_tree_mean<<<grid_of(nseg), kTreeBlock, 0, st>>>(seg, nseg, x, int2pos, mean, split);
+    k_tree_mean<<<(unsigned)((nseg + kTreeBlock / 64 - 1) / (kTreeBlock / 64)), kTreeBlock, 0, st>>>(
+        seg, nseg, x, int2pos, mean, split);
     k_tree_flags<<<grid_of(n), kTreeBlock, 0, st>>>(x, w.segof, seg, mean, int2pos, n, w.small);
     hipError_t e = rocprim::exclusive_scan(temp, *temp_bytes, w.small, w.sx, 0, (size_t)n, rocprim::plus<int>(), st);
     if (e != hipSuccess)
