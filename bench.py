@@ -383,14 +383,43 @@ def main():
         tt, ts = ctx.get_tasks()
         same_walk2 = bool(np.array_equal(tt, w.task_t) and np.array_equal(ts, w.task_s))
         del tt, ts
+
+        # one whole P2P step from grid positions in input order, steady state:
+        # (a) all on the device: tree, walk, CSR + work queue, force;
+        # (b) host tree and walk (timed above), then positions and list uploaded,
+        #     CSR + work queue, force
+        def device_step():
+            ctx.build_tree(posi_in, tag_in, m["int2pos"], m["maxleaf"], m["bdl"], m["bdr"])
+            ctx.walk_tree(np.float32(m["open_angle"]), np.float32(m["cutoff_radius"]))
+            ctx.prepare()
+            ctx.zero_acc()
+            ctx.compute(exec_form)
+            ctx.sync()
+
+        def upload_step():
+            ctx.set_particles_i32(m["posi"], m["int2pos"], w.leaf_ip, w.leaf_np)
+            ctx.set_tasks(w.task_t, w.task_s)
+            ctx.prepare()
+            ctx.zero_acc()
+            ctx.compute(exec_form)
+            ctx.sync()
+
+        _, dev_step_ms = best_of(device_step)
+        _, up_step_ms = best_of(upload_step)
         lists = {"host_tree_s": ls["tree"], "host_walk_s": ls["walk"], "host_threads": threads,
                  "device_walk_ms": walk_ms, "device_walk_identical": same_walk,
                  "device_tree_ms": tree_ms, "device_tree_identical": same_tree, "set_particles_i32_ms": upload_ms,
-                 "device_tree_walk_ms": walk2_ms, "device_tree_walk_identical": same_walk2, "ntask": int(nwalk)}
+                 "device_tree_walk_ms": walk2_ms, "device_tree_walk_identical": same_walk2, "ntask": int(nwalk),
+                 "step_all_device_ms": dev_step_ms,
+                 "step_host_lists_ms": 1e3 * (ls["tree"] + ls["walk"]) + up_step_ms,
+                 "step_host_lists_upload_and_device_ms": up_step_ms}
         log(f"[bench] lists: host tree {ls['tree']:.2f}s + walk {ls['walk']:.2f}s on {threads} threads; "
             f"device walk of the host tree {walk_ms:.1f} ms (identical: {same_walk}); device tree {tree_ms:.1f} ms "
             f"(identical: {same_tree}; the position upload alone {upload_ms:.1f} ms) + walk {walk2_ms:.1f} ms "
             f"(identical: {same_walk2})")
+        log(f"[bench] one step from grid positions: all on the device {dev_step_ms:.1f} ms; host tree + walk, "
+            f"then upload + prepare + force {1e3 * (ls['tree'] + ls['walk']) + up_step_ms:.1f} ms "
+            f"({up_step_ms:.1f} ms after the host lists)")
     ctx.close()
 
     copy_cal = None
