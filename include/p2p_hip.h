@@ -14,7 +14,7 @@
  * PROC_RANK, PROC_SIZE, task_t, task_s), which the host program defines
  * (src/global.c in the reference).
  *
- * Deviations: errors are recorded and reported by
+ * Deviations from the reference: errors are recorded and reported by
  * p2p_hip_status() instead of exit(); the caller's task arrays are not
  * rebased in place; the launch for a step is issued at the last_time call
  * (one persistent launch over all flushes); get_last_kntime returns the
