@@ -105,6 +105,23 @@ void oracle_count_pairs(const oracle_params *p, const double *pos,
                         int active_level, long long *n_list, long long *n_in,
                         int nthreads);
 
+/*
+ * Short-range multipole operators (oracle_multipole.c): the reference's M2L
+ * (src/operator.c:331-474, LONGSHORT, up to octupole) and L2P (:203-262) in
+ * its FP32 arithmetic, and an independent FP64 form of M2L, L2P and P2M.
+ * Coefficient layout of inc/operator.h, ORACLE_NMULTI = 20.
+ */
+#define ORACLE_NMULTI 20
+void oracle_m2l_ref32(float dx, float dy, float dz, const float *M, float rs, float *L);
+void oracle_l2p_ref32(int npart, const int *posi, double int2pos, const float *center, const float *L,
+                      float *acc_x, float *acc_y, float *acc_z);
+void oracle_radial_f64(double r, double rs, double *f);
+void oracle_m2l_f64(double dx, double dy, double dz, const double *M, double rs, double *L);
+void oracle_l2p_f64(double dx, double dy, double dz, const double *L, double *F);
+void oracle_p2m_f64(int n, const double *pos, double mass, const double *center, double *M);
+void oracle_m2l_tasks_f64(int ncell, const double *center, const double *M, long long ntask, const int *task_t,
+                          const int *task_s, double rs, double *L);
+
 #ifdef __cplusplus
 }
 #endif

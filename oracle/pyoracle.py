@@ -53,6 +53,17 @@ def load() -> ctypes.CDLL:
     lib.oracle_build_csr.argtypes = [i64, vp, vp, i32, vp, vp, vp]
     lib.oracle_count_pairs.restype = None
     lib.oracle_count_pairs.argtypes = common + [ctypes.POINTER(i64), ctypes.POINTER(i64), i32]
+    flt = ctypes.c_float
+    for name, args in (("oracle_m2l_ref32", [flt, flt, flt, vp, flt, vp]),
+                       ("oracle_l2p_ref32", [i32, vp, dbl, vp, vp, vp, vp, vp]),
+                       ("oracle_radial_f64", [dbl, dbl, vp]),
+                       ("oracle_m2l_f64", [dbl, dbl, dbl, vp, dbl, vp]),
+                       ("oracle_l2p_f64", [dbl, dbl, dbl, vp, vp]),
+                       ("oracle_p2m_f64", [i32, vp, dbl, vp, vp]),
+                       ("oracle_m2l_tasks_f64", [i32, vp, vp, i64, vp, vp, dbl, vp])):
+        fn = getattr(lib, name)
+        fn.restype = None
+        fn.argtypes = args
     _lib = lib
     return lib
 
@@ -185,3 +196,71 @@ def reflist_build(posi, tag, int2pos, maxleaf, open_angle, cutoff_radius, bdl, b
     for p in (perm, lip, lnp, tt, ts):
         lib.reflist_free(ctypes.cast(p, vp))
     return out
+
+
+# ---------------------------------------------------------------- multipoles
+
+NMULTI = 20
+
+
+def m2l_ref32(d, M, rs, L0):
+    """The reference's FP32 M2L (src/operator.c:331-474) for each row of
+    d (n x 3), M (n x 20), rs (n), added to L0 (n x 20)."""
+    lib = load()
+    d = np.asarray(d, np.float32)
+    M = np.ascontiguousarray(M, np.float32)
+    L = np.array(L0, np.float32, copy=True)
+    rs = np.broadcast_to(np.asarray(rs, np.float32), (d.shape[0],))
+    for k in range(d.shape[0]):
+        lib.oracle_m2l_ref32(float(d[k, 0]), float(d[k, 1]), float(d[k, 2]), _p(M[k]), float(rs[k]), _p(L[k]))
+    return L
+
+
+def l2p_ref32(posi, int2pos, center, L, acc0):
+    """The reference's FP32 L2P (src/operator.c:203-262) of one leaf: acc0
+    (3 x n) plus the forces at the n particles posi (n x 3 int32)."""
+    lib = load()
+    posi = np.ascontiguousarray(posi, np.int32)
+    acc = [np.array(acc0[d], np.float32, copy=True) for d in range(3)]
+    lib.oracle_l2p_ref32(posi.shape[0], _p(posi), float(int2pos), _p(np.ascontiguousarray(center, np.float32)),
+                         _p(np.ascontiguousarray(L, np.float32)), _p(acc[0]), _p(acc[1]), _p(acc[2]))
+    return np.stack(acc)
+
+
+def radial_f64(r, rs):
+    f = np.zeros(4)
+    load().oracle_radial_f64(float(r), float(rs), _p(f))
+    return f
+
+
+def m2l_f64(d, M, rs, L0=None):
+    """FP64 M2L of one separation d (3) and multipole M (20), added to L0."""
+    L = np.zeros(NMULTI) if L0 is None else np.array(L0, np.float64, copy=True)
+    load().oracle_m2l_f64(float(d[0]), float(d[1]), float(d[2]), _p(np.ascontiguousarray(M, np.float64)),
+                          float(rs), _p(L))
+    return L
+
+
+def l2p_f64(d, L):
+    F = np.zeros(3)
+    load().oracle_l2p_f64(float(d[0]), float(d[1]), float(d[2]), _p(np.ascontiguousarray(L, np.float64)), _p(F))
+    return F
+
+
+def p2m_f64(pos, mass, center):
+    pos = np.ascontiguousarray(pos, np.float64)
+    M = np.zeros(NMULTI)
+    load().oracle_p2m_f64(pos.shape[0], _p(pos), float(mass), _p(np.ascontiguousarray(center, np.float64)), _p(M))
+    return M
+
+
+def m2l_tasks_f64(center, M, task_t, task_s, rs, L0=None):
+    """The M2L task loop (src/fmm.c:1188-1217) in FP64 over cells with
+    centres center (n x 3) and multipoles M (n x 20)."""
+    center = np.ascontiguousarray(center, np.float64)
+    M = np.ascontiguousarray(M, np.float64)
+    L = np.zeros_like(M) if L0 is None else np.array(L0, np.float64, copy=True)
+    tt = np.ascontiguousarray(task_t, np.int32)
+    ts = np.ascontiguousarray(task_s, np.int32)
+    load().oracle_m2l_tasks_f64(center.shape[0], _p(center), _p(M), tt.size, _p(tt), _p(ts), float(rs), _p(L))
+    return L

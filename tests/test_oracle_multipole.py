@@ -1,0 +1,167 @@
+"""Pins of the multipole oracle (oracle/oracle_multipole.c), CPU only.
+
+* The FP32 restatement of the reference's M2L and L2P equals the reference's
+  own compiled functions bit for bit (tests/golden/m2l_ref.npz, written by
+  tests/golden/make_golden_m2l.py from oracle/_ref), including the skip of
+  every separation with r^2 < 1 (the reference tests abs() of an int).
+* The FP64 form is derived independently: its radial derivatives and the
+  derivative tensors of a point mass match finite differences of
+  phi(r) = erfc(r / 2 rs) / r; it agrees with the reference's FP32 closed
+  forms to FP32 rounding; and P2M -> M2L -> L2P of a cluster converges to the
+  direct sum of the erfc forces at the octupole truncation order."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "m2l_ref.npz")
+EXP = [(0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1), (2, 0, 0), (1, 1, 0), (1, 0, 1), (0, 2, 0), (0, 1, 1), (0, 0, 2),
+       (3, 0, 0), (2, 1, 0), (2, 0, 1), (1, 2, 0), (1, 1, 1), (1, 0, 2), (0, 3, 0), (0, 2, 1), (0, 1, 2), (0, 0, 3)]
+
+
+def phi(x, rs):
+    r = np.linalg.norm(x)
+    return math.erfc(r / (2 * rs)) / r
+
+
+def fd_deriv(e, x, rs, h):
+    """Central finite difference of phi with derivative orders e at x."""
+    def rec(e, x):
+        for d in range(3):
+            if e[d]:
+                e2 = list(e)
+                e2[d] -= 1
+                xp, xm = x.copy(), x.copy()
+                xp[d] += h
+                xm[d] -= h
+                return (rec(e2, xp) - rec(e2, xm)) / (2 * h)
+        return phi(x, rs)
+    return rec(list(e), np.asarray(x, np.float64))
+
+
+def test_m2l_ref32_equals_reference_bitwise():
+    f = np.load(GOLDEN)
+    L = po.m2l_ref32(f["m2l_d"], f["m2l_M"], f["m2l_rs"], f["m2l_L0"])
+    np.testing.assert_array_equal(L, f["m2l_L"])
+    r2 = (f["m2l_d"].astype(np.float64) ** 2).sum(1)
+    skipped = np.all(f["m2l_L"] == f["m2l_L0"], axis=1)
+    assert np.array_equal(skipped, r2 < 1.0)          # the reference's int abs(): r^2 < 1 adds nothing
+
+
+def test_l2p_ref32_equals_reference_bitwise():
+    f = np.load(GOLDEN)
+    acc = f["l2p_acc0"].copy()
+    for l in range(f["l2p_ip"].size):
+        a, n = f["l2p_ip"][l], f["l2p_np"][l]
+        acc[:, a:a + n] = po.l2p_ref32(f["l2p_posi"][a:a + n], f["l2p_int2pos"], f["l2p_center"][l], f["l2p_L"][l],
+                                       acc[:, a:a + n])
+    np.testing.assert_array_equal(acc, f["l2p_acc"])
+
+
+@pytest.mark.parametrize("r,rs", [(1.3, 1.0), (3.7, 1.0), (5.0, 1.953125), (2.2, 0.7)])
+def test_radial_derivatives_match_finite_differences(r, rs):
+    f = po.radial_f64(r, rs)
+    h = 1e-4 * r
+    g = lambda rr: math.erfc(rr / (2 * rs)) / rr  # noqa: E731
+    assert f[0] == pytest.approx(g(r), rel=1e-14)
+    # f[k+1] = (1/r) d f[k] / dr
+    fk = [lambda rr, k=k: po.radial_f64(rr, rs)[k] for k in range(3)]
+    for k in range(3):
+        d = (fk[k](r + h) - fk[k](r - h)) / (2 * h) / r
+        assert f[k + 1] == pytest.approx(d, rel=1e-6), k
+
+
+@pytest.mark.parametrize("x,rs", [((2.0, -1.1, 0.7), 1.0), ((0.3, 3.1, -2.2), 1.953125), ((-1.7, 0.4, 1.9), 0.7)])
+def test_point_mass_tensors_match_finite_differences(x, rs):
+    """M = unit point mass: L holds the potential and its derivatives up to
+    third order at x, each checked against finite differences of phi."""
+    M = np.zeros(20)
+    M[0] = 1.0
+    L = po.m2l_f64(x, M, rs)
+    r = np.linalg.norm(x)
+    for k, e in enumerate(EXP):
+        n = sum(e)
+        h = 5e-3 * r
+        # Richardson: the O(h^2) error of the central differences cancels
+        fd = (4 * fd_deriv(e, np.array(x), rs, h / 2) - fd_deriv(e, np.array(x), rs, h)) / 3
+        scale = max(abs(fd), 1e-3 * abs(L[0]) / r ** n)
+        assert abs(L[k] - fd) <= 2e-6 * scale, (k, e, L[k], fd)
+
+
+def test_f64_agrees_with_reference_closed_forms():
+    """The FP64 form (derived here) against the reference's FP32 closed forms
+    on the golden separations with r^2 >= 1: FP32 rounding apart."""
+    f = np.load(GOLDEN)
+    ok = (f["m2l_d"].astype(np.float64) ** 2).sum(1) >= 1.0
+    # the FP32 restatement (bit-equal to the reference, pinned above) from
+    # L = 0, so that no rounding of the golden L0 hides the contributions
+    ref32 = po.m2l_ref32(f["m2l_d"], f["m2l_M"], f["m2l_rs"], np.zeros_like(f["m2l_L0"]))
+    worst = 0.0
+    for k in np.nonzero(ok)[0]:
+        L = po.m2l_f64(f["m2l_d"][k].astype(np.float64), f["m2l_M"][k].astype(np.float64), float(f["m2l_rs"][k]))
+        ref = ref32[k].astype(np.float64)
+        # per order: error relative to the largest coefficient of that order
+        for lo, hi in ((0, 1), (1, 4), (4, 10), (10, 20)):
+            s = np.abs(L[lo:hi]).max()
+            if s > 1e-30:
+                worst = max(worst, np.abs(L[lo:hi] - ref[lo:hi]).max() / s)
+    assert worst < 2e-4, worst
+
+
+def _direct_force(t, pos, mass, rs):
+    d = t[None, :] - pos
+    r = np.linalg.norm(d, axis=1)
+    u = r / (2 * rs)
+    erfcs = np.array([math.erfc(v) for v in u])
+    dphi = -(erfcs / r ** 2 + np.exp(-u * u) / (math.sqrt(math.pi) * rs * r))      # d phi / d r
+    return (mass * dphi[:, None] * d / r[:, None]).sum(0)
+
+
+@pytest.mark.parametrize("rs", [1.0, 2.0])
+def test_p2m_m2l_l2p_converges_at_octupole_order(rs):
+    """A cluster of radius a at distance R, its force at targets within a of
+    the local centre: expansion terms up to total order 3 in the source and
+    target offsets leave a potential error of order a^4, a force error of
+    order a^3 (1/8 per halving of a, same configuration scaled)."""
+    g = np.random.default_rng(int(rs * 10))
+    unit_src = g.uniform(-1, 1, (30, 3))
+    unit_tgt = g.uniform(-1, 1, (8, 3))
+    # at 2.3 rs: the erfc kernel varies on the scale 2 rs^2 / r near the
+    # cutoff, where the expansion converges more slowly than for 1/r
+    R = np.array([2.0, 0.8, -0.6]) * rs
+    errs = []
+    for a in (0.1 * rs, 0.05 * rs):
+        pos = a * unit_src
+        tgt = a * unit_tgt
+        M = po.p2m_f64(pos, 0.7, np.zeros(3))
+        L = po.m2l_f64(R, M, rs)
+        e = 0.0
+        for t in tgt:
+            F = po.l2p_f64(t, L)
+            ref = _direct_force(R + t, pos, 0.7, rs)
+            e = max(e, np.linalg.norm(F - ref) / np.linalg.norm(ref))
+        errs.append(e)
+    assert errs[0] < 2e-3
+    assert errs[0] / 10 < errs[1] < errs[0] / 6
+
+
+def test_m2l_task_loop_sums_tasks_and_skips_close_pairs():
+    g = np.random.default_rng(5)
+    n = 12
+    center = g.uniform(0, 10, (n, 3))
+    center[3] = center[2] + 0.3                       # a pair closer than 1: skipped
+    M = g.normal(size=(n, 20)) * 0.1
+    M[:, 0] = 1.0
+    tt = np.array([0, 0, 1, 2, 5, 7, 0], np.int32)
+    ts = np.array([1, 4, 0, 3, 6, 8, 1], np.int32)
+    L = po.m2l_tasks_f64(center, M, tt, ts, 1.5)
+    ref = np.zeros_like(M)
+    for t, s in zip(tt, ts):
+        d = center[t] - center[s]
+        if d @ d >= 1.0:
+            ref[t] = po.m2l_f64(d, M[s], 1.5, ref[t])
+    np.testing.assert_array_equal(L, ref)
+    assert np.all(L[2] == 0.0)
