@@ -183,6 +183,27 @@ int pp2p_get_tree(pp2p_ctx *ctx, int *perm, int *leaf_ip, int *leaf_np, int *lea
  * (nullable) receives its length.  Synchronous. */
 int pp2p_get_tasks(pp2p_ctx *ctx, long long *ntask, int *task_t, int *task_s);
 
+/*
+ * The short-range M2L task loop (task_compute_m2l, src/fmm.c:1188-1217) with
+ * the erfc-modified kernel of the long/short split (m2l, src/operator.c:
+ * 331-474, up to octupole): for each task k, L[task_t[k]] += the local
+ * expansion at center[task_t[k]] of the multipole M[task_s[k]] about
+ * center[task_s[k]], for split radius rs.  ncell cells (nodes and leaves in
+ * one index space); center ncell x 3, M and L ncell x 20 in the reference's
+ * coefficient layout (inc/operator.h, NMULTI = 20), FP32.  L is read and
+ * written (it accumulates, as the reference's toL += ...).  Separations with
+ * r^2 < 1 add nothing, as in the reference (its abs() of an int).
+ * *kernel_ms (nullable) receives the kernel's device time.  Synchronous;
+ * independent of the context's particles and P2P list.
+ */
+int pp2p_m2l(pp2p_ctx *ctx, int ncell, const float *center, const float *M, long long ntask, const int *task_t,
+             const int *task_s, float rs, float *L, double *kernel_ms);
+/* L2P (l2p, src/operator.c:203-262): the force of each leaf's local
+ * expansion L (nleaf x 20, about center, nleaf x 3) at the leaf's particles,
+ * added to the accumulator (pp2p_get_acc_*), for the context's particles and
+ * leaves.  Synchronous. */
+int pp2p_l2p(pp2p_ctx *ctx, const float *L, const float *center);
+
 /* Builds the target CSR and the cost-sorted work queue of the current list
  * on the device.  pp2p_compute calls it when the list changed. */
 int pp2p_prepare(pp2p_ctx *ctx);

@@ -14,6 +14,7 @@
 #include "forcelaw.h"
 #include "p2p_build.h"
 #include "p2p_launch.h"
+#include "p2p_m2l.h"
 #include "p2p_tree.h"
 #include "p2p_walk.h"
 
@@ -156,6 +157,12 @@ struct pp2p_ctx {
     DevBuf<int> tnodef, tleaff, tnoff, tloff;
     bool tree_ready = false;    // walk arrays, tidx and rng hold the last pp2p_build_tree
 
+    // M2L task loop and L2P: cells, their expansions, the task CSR
+    DevBuf<float4> mcenter;
+    DevBuf<float> mM, mL;
+    DevBuf<int> mtt, mts, mkey, msrc, mflag, mrowidx, mrow_tgt, mrow_off;
+    hipEvent_t ev_m0 = nullptr, ev_m1 = nullptr;
+
     float last_kernel_ms = 0.f, last_prepare_ms = 0.f;
     bool kernel_timed = false, prepare_timed = false;
 };
@@ -245,6 +252,11 @@ void pp2p_destroy(pp2p_ctx *c)
     c->tx0.release(); c->tx1.release(); c->tx2.release(); c->tidx.release(); c->tsegof.release();
     c->tsmall.release(); c->tsx.release(); c->ttbl.release(); c->ttag.release(); c->tposi.release();
     c->tnodef.release(); c->tleaff.release(); c->tnoff.release(); c->tloff.release();
+    c->mcenter.release(); c->mM.release(); c->mL.release();
+    for (DevBuf<int> *b : {&c->mtt, &c->mts, &c->mkey, &c->msrc, &c->mflag, &c->mrowidx, &c->mrow_tgt, &c->mrow_off})
+        b->release();
+    if (c->ev_m0) (void)hipEventDestroy(c->ev_m0);
+    if (c->ev_m1) (void)hipEventDestroy(c->ev_m1);
     c->idx.release(); c->idx_sorted.release(); c->counter.release(); c->temp.release();
     c->pair_count.release();
     if (c->ev_k0) (void)hipEventDestroy(c->ev_k0);
@@ -830,6 +842,97 @@ int pp2p_get_tree(pp2p_ctx *c, int *perm, int *leaf_ip, int *leaf_np, int *leaf_
         (node_center && (st = copy_xyz(c, c->wnc.p, nnode, node_center))) ||
         (node_width && (st = copy_xyz(c, c->wnw.p, nnode, node_width))))
         return st;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return PP2P_OK;
+}
+
+static int upload_centers(pp2p_ctx *c, int n, const float *xyz)
+{
+    return upload_boxes(c, c->mcenter, n, xyz);
+}
+
+int pp2p_m2l(pp2p_ctx *c, int ncell, const float *center, const float *M, long long ntask, const int *task_t,
+             const int *task_s, float rs, float *L, double *kernel_ms)
+{
+    int st = check_ctx(c);
+    if (st)
+        return st;
+    if (ncell < 1 || !center || !M || !L || ntask < 0 || (ntask > 0 && (!task_t || !task_s)))
+        return fail(PP2P_ERR_ARG, "invalid cell or task arrays");
+    if (!(rs > 0.f) || std::isinf(rs))
+        return fail(PP2P_ERR_ARG, "invalid split radius");
+    if (ntask > (1ll << 31) - 1 || (long long)ncell * 20 > (1ll << 31) - 1)
+        return fail(PP2P_ERR_ARG, "more than 2^31-1 tasks or coefficients");
+    for (long long k = 0; k < ntask; ++k)
+        if (task_t[k] < 0 || task_t[k] >= ncell || task_s[k] < 0 || task_s[k] >= ncell)
+            return fail(PP2P_ERR_ARG, "m2l task " + std::to_string(k) + " names a cell outside [0, ncell)");
+    const hipStream_t sm = c->stream;
+    if ((st = upload_centers(c, ncell, center)))
+        return st;
+    const size_t nco = (size_t)20 * ncell;
+    HIPCHK(c->mM.reserve(nco));
+    HIPCHK(c->mL.reserve(nco));
+    HIPCHK(hipMemcpyAsync(c->mM.p, M, sizeof(float) * nco, hipMemcpyHostToDevice, sm));
+    HIPCHK(hipMemcpyAsync(c->mL.p, L, sizeof(float) * nco, hipMemcpyHostToDevice, sm));
+    const int n = (int)ntask;
+    int nrows = 0;
+    if (n > 0) {
+        for (DevBuf<int> *b : {&c->mtt, &c->mts, &c->mkey, &c->msrc, &c->mflag, &c->mrowidx, &c->mrow_tgt})
+            HIPCHK(b->reserve(n));
+        HIPCHK(c->mrow_off.reserve(n + 1));
+        HIPCHK(hipMemcpyAsync(c->mtt.p, task_t, sizeof(int) * n, hipMemcpyHostToDevice, sm));
+        HIPCHK(hipMemcpyAsync(c->mts.p, task_s, sizeof(int) * n, hipMemcpyHostToDevice, sm));
+        int bits = 1;
+        while ((1ll << bits) < (long long)ncell)
+            ++bits;
+        size_t tb1 = 0, tb2 = 0;
+        HIPCHK(csr_sort(c->mtt.p, c->mts.p, c->mkey.p, c->msrc.p, n, bits, nullptr, &tb1, sm));
+        HIPCHK(csr_rows(c->mkey.p, n, c->mflag.p, c->mrowidx.p, c->mrow_tgt.p, c->mrow_off.p, nullptr, &tb2, sm));
+        if ((st = ensure_temp(c, std::max(tb1, tb2))))
+            return st;
+        tb1 = tb2 = c->temp.cap;
+        HIPCHK(csr_sort(c->mtt.p, c->mts.p, c->mkey.p, c->msrc.p, n, bits, c->temp.p, &tb1, sm));
+        HIPCHK(csr_rows(c->mkey.p, n, c->mflag.p, c->mrowidx.p, c->mrow_tgt.p, c->mrow_off.p, c->temp.p, &tb2,
+                        sm));
+        HIPCHK(hipMemcpyAsync(&nrows, c->mrowidx.p + n - 1, sizeof(int), hipMemcpyDeviceToHost, sm));
+        HIPCHK(hipStreamSynchronize(sm));
+    }
+    if (!c->ev_m0) {
+        HIPCHK(hipEventCreate(&c->ev_m0));
+        HIPCHK(hipEventCreate(&c->ev_m1));
+    }
+    const M2LArgs a{c->mcenter.p, c->mM.p, c->mrow_tgt.p, c->mrow_off.p, c->msrc.p, nrows, rs, c->mL.p};
+    HIPCHK(hipEventRecord(c->ev_m0, sm));
+    HIPCHK(launch_m2l(a, sm));
+    HIPCHK(hipEventRecord(c->ev_m1, sm));
+    HIPCHK(hipMemcpyAsync(L, c->mL.p, sizeof(float) * nco, hipMemcpyDeviceToHost, sm));
+    HIPCHK(hipStreamSynchronize(sm));
+    if (kernel_ms) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, c->ev_m0, c->ev_m1));
+        *kernel_ms = ms;
+    }
+    return PP2P_OK;
+}
+
+int pp2p_l2p(pp2p_ctx *c, const float *L, const float *center)
+{
+    int st = check_ctx(c);
+    if (st)
+        return st;
+    if (!c->have_particles)
+        return fail(PP2P_ERR_STATE, "set_particles before l2p");
+    if (c->nleaf > 0 && (!L || !center))
+        return fail(PP2P_ERR_ARG, "invalid expansion arrays");
+    if (c->nleaf == 0)
+        return PP2P_OK;
+    if ((st = upload_centers(c, c->nleaf, center)))
+        return st;
+    const size_t nco = (size_t)20 * c->nleaf;
+    HIPCHK(c->mL.reserve(nco));
+    HIPCHK(hipMemcpyAsync(c->mL.p, L, sizeof(float) * nco, hipMemcpyHostToDevice, c->stream));
+    const L2PArgs a{c->rng.p, c->mcenter.p, c->mL.p, c->x3.p, c->nleaf, c->npart, c->acc.p};
+    HIPCHK(launch_l2p(a, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return PP2P_OK;
 }

@@ -267,6 +267,18 @@ struct Walker {
     const Tree &t;
     float open_angle, cutoff;
     std::vector<int> *tt, *ts;
+    // well-separated pairs (acceptance 1), the tasks of walk_task_m2l
+    // (src/fmm.c:812-...): cells as leaf id, or nleaf + node id; null: not kept
+    std::vector<int> *mt = nullptr, *ms = nullptr;
+
+    void emit_m2l(int im, int jm)
+    {
+        if (!mt)
+            return;
+        const int nl = (int)t.leaf.size();
+        mt->push_back(im < 0 ? -im - 1 : nl + im);
+        ms->push_back(jm < 0 ? -jm - 1 : nl + jm);
+    }
 
     // a: >= 0 node id, < 0 leaf -(id+1)
     void emit(int lt, int ls)
@@ -301,9 +313,12 @@ struct Walker {
             const Node &b = t.node[jm];
             for (int d = 0; d < 3; ++d)
                 dist[d] = a.center[d] - b.center[d];
-            if (accept(a.width, b.width, dist, open_angle, cutoff) == 0) {
+            const int f = accept(a.width, b.width, dist, open_angle, cutoff);
+            if (f == 0) {
                 walk(im, b.son[0]);
                 walk(im, b.son[1]);
+            } else if (f == 1) {
+                emit_m2l(im, jm);
             }
             return;
         }
@@ -312,9 +327,12 @@ struct Walker {
             const Leaf &b = t.leaf[-jm - 1];
             for (int d = 0; d < 3; ++d)
                 dist[d] = a.center[d] - b.center[d];
-            if (accept(b.width, a.width, dist, open_angle, cutoff) == 0) {
+            const int f = accept(b.width, a.width, dist, open_angle, cutoff);
+            if (f == 0) {
                 walk(a.son[0], jm);
                 walk(a.son[1], jm);
+            } else if (f == 1) {
+                emit_m2l(im, jm);
             }
             return;
         }
@@ -322,7 +340,8 @@ struct Walker {
         const Node &b = t.node[jm];
         for (int d = 0; d < 3; ++d)
             dist[d] = a.center[d] - b.center[d];
-        if (accept(a.width, b.width, dist, open_angle, cutoff) == 0) {
+        const int f = accept(a.width, b.width, dist, open_angle, cutoff);
+        if (f == 0) {
             if (a.width[0] + a.width[1] + a.width[2] > b.width[0] + b.width[1] + b.width[2]) {
                 walk(a.son[0], jm);
                 walk(a.son[1], jm);
@@ -330,6 +349,8 @@ struct Walker {
                 walk(im, b.son[0]);
                 walk(im, b.son[1]);
             }
+        } else if (f == 1) {
+            emit_m2l(im, jm);
         }
     }
 };
@@ -365,10 +386,13 @@ void expand(const Tree &t, float open_angle, float cutoff, std::vector<std::pair
                 const Node &b = t.node[jm];
                 for (int d = 0; d < 3; ++d)
                     dist[d] = a.center[d] - b.center[d];
-                if (accept(a.width, b.width, dist, open_angle, cutoff) == 0) {
+                const int fl = accept(a.width, b.width, dist, open_angle, cutoff);
+                if (fl == 0) {
                     next.push_back({im, b.son[0]});
                     next.push_back({im, b.son[1]});
                     grew = true;
+                } else if (fl == 1) {
+                    next.push_back(f);          // a well-separated pair: the walk records it
                 }
                 continue;
             }
@@ -377,10 +401,13 @@ void expand(const Tree &t, float open_angle, float cutoff, std::vector<std::pair
                 const Leaf &b = t.leaf[-jm - 1];
                 for (int d = 0; d < 3; ++d)
                     dist[d] = a.center[d] - b.center[d];
-                if (accept(b.width, a.width, dist, open_angle, cutoff) == 0) {
+                const int fl = accept(b.width, a.width, dist, open_angle, cutoff);
+                if (fl == 0) {
                     next.push_back({a.son[0], jm});
                     next.push_back({a.son[1], jm});
                     grew = true;
+                } else if (fl == 1) {
+                    next.push_back(f);
                 }
                 continue;
             }
@@ -388,7 +415,8 @@ void expand(const Tree &t, float open_angle, float cutoff, std::vector<std::pair
             const Node &b = t.node[jm];
             for (int d = 0; d < 3; ++d)
                 dist[d] = a.center[d] - b.center[d];
-            if (accept(a.width, b.width, dist, open_angle, cutoff) == 0) {
+            const int fl = accept(a.width, b.width, dist, open_angle, cutoff);
+            if (fl == 0) {
                 if (a.width[0] + a.width[1] + a.width[2] > b.width[0] + b.width[1] + b.width[2]) {
                     next.push_back({a.son[0], jm});
                     next.push_back({a.son[1], jm});
@@ -397,6 +425,8 @@ void expand(const Tree &t, float open_angle, float cutoff, std::vector<std::pair
                     next.push_back({im, b.son[1]});
                 }
                 grew = true;
+            } else if (fl == 1) {
+                next.push_back(f);
             }
         }
         frames.swap(next);
@@ -412,6 +442,9 @@ struct phost_lists {
     // task pairs per walk frame, in frame order; off[k] = tasks before frame k
     std::vector<std::vector<int>> ft, fs;
     std::vector<long long> off;
+    // well-separated pairs per walk frame (cells: leaf id, or nleaf + node id)
+    std::vector<std::vector<int>> mt, ms;
+    std::vector<long long> moff{0};
     int nthreads = 1;
 };
 
@@ -469,14 +502,19 @@ int phost_walk(phost_lists *h, float open_angle, float cutoff_radius, int nthrea
     expand(t, open_angle, cutoff_radius, frames, (size_t)nthreads * 64);
     h->ft.assign(frames.size(), {});
     h->fs.assign(frames.size(), {});
+    h->mt.assign(frames.size(), {});
+    h->ms.assign(frames.size(), {});
 #pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
     for (long long k = 0; k < (long long)frames.size(); ++k) {
-        Walker w{t, open_angle, cutoff_radius, &h->ft[k], &h->fs[k]};
+        Walker w{t, open_angle, cutoff_radius, &h->ft[k], &h->fs[k], &h->mt[k], &h->ms[k]};
         w.walk(frames[k].first, frames[k].second);
     }
     h->off.assign(frames.size() + 1, 0);
-    for (size_t k = 0; k < frames.size(); ++k)
+    h->moff.assign(frames.size() + 1, 0);
+    for (size_t k = 0; k < frames.size(); ++k) {
         h->off[k + 1] = h->off[k] + (long long)h->ft[k].size();
+        h->moff[k + 1] = h->moff[k] + (long long)h->mt[k].size();
+    }
     h->nthreads = nthreads;
     return 0;
 }
@@ -500,6 +538,7 @@ long long phost_count(const phost_lists *h, int what)
     case PHOST_NLEAF: return (long long)h->tree.leaf.size();
     case PHOST_NNODE: return (long long)h->tree.node.size();
     case PHOST_NTASK: return h->off.back();
+    case PHOST_NM2L: return h->moff.back();
     default: return -1;
     }
 }
@@ -542,6 +581,15 @@ void phost_get_tasks(const phost_lists *h, int *task_t, int *task_s)
     for (long long k = 0; k < (long long)h->ft.size(); ++k) {
         std::memcpy(task_t + h->off[k], h->ft[k].data(), sizeof(int) * h->ft[k].size());
         std::memcpy(task_s + h->off[k], h->fs[k].data(), sizeof(int) * h->fs[k].size());
+    }
+}
+
+void phost_get_m2l(const phost_lists *h, int *task_t, int *task_s)
+{
+#pragma omp parallel for schedule(dynamic, 1) num_threads(h->nthreads)
+    for (long long k = 0; k < (long long)h->mt.size(); ++k) {
+        std::memcpy(task_t + h->moff[k], h->mt[k].data(), sizeof(int) * h->mt[k].size());
+        std::memcpy(task_s + h->moff[k], h->ms[k].data(), sizeof(int) * h->ms[k].size());
     }
 }
 

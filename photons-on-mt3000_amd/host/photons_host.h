@@ -8,7 +8,7 @@
 extern "C" {
 #endif
 
-enum { PHOST_NPART = 0, PHOST_NLEAF = 1, PHOST_NNODE = 2, PHOST_NTASK = 3 };
+enum { PHOST_NPART = 0, PHOST_NLEAF = 1, PHOST_NNODE = 2, PHOST_NTASK = 3, PHOST_NM2L = 4 };
 
 typedef struct phost_lists phost_lists;
 
@@ -32,6 +32,10 @@ void phost_get_leaves(const phost_lists *h, int *ip, int *np, int *active, float
  * -(id + 1)), center/width nnode x 3 */
 void phost_get_nodes(const phost_lists *h, int *son, float *center, float *width);
 void phost_get_tasks(const phost_lists *h, int *task_t, int *task_s);
+/* The well-separated cell pairs the walk met (acceptance 1: the M2L tasks of
+ * walk_task_m2l, src/fmm.c:812-900), in walk order; cells are numbered leaf
+ * id, or nleaf + node id. */
+void phost_get_m2l(const phost_lists *h, int *task_t, int *task_s);
 void phost_free(phost_lists *h);
 
 #ifdef __cplusplus

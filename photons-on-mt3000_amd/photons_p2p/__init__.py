@@ -60,6 +60,7 @@ CORE_SYMBOLS = (
     "pp2p_set_params", "pp2p_set_tuning", "pp2p_get_tuning", "pp2p_set_particles_f64",
     "pp2p_set_particles_i32", "pp2p_set_active", "pp2p_set_tasks", "pp2p_add_tasks",
     "pp2p_walk_tasks", "pp2p_get_tasks", "pp2p_build_tree", "pp2p_walk_tree", "pp2p_get_tree",
+    "pp2p_m2l", "pp2p_l2p",
     "pp2p_prepare", "pp2p_compute", "pp2p_zero_acc", "pp2p_run_graph", "pp2p_sync",
     "pp2p_get_acc_f64", "pp2p_get_acc_f32_soa", "pp2p_list_stats", "pp2p_last_kernel_ms",
     "pp2p_last_prepare_ms", "pp2p_count_inside", "pp2p_time_steps", "pp2p_law_eval",
@@ -100,6 +101,8 @@ def load(path: str | None = None) -> ctypes.CDLL:
         "pp2p_build_tree": (i32, [vp, i64, vp, vp, dbl, i32, vp, vp, ctypes.POINTER(i32), ctypes.POINTER(i32)]),
         "pp2p_walk_tree": (i32, [vp, ctypes.c_float, ctypes.c_float, ctypes.POINTER(i64)]),
         "pp2p_get_tree": (i32, [vp] + [vp] * 9),
+        "pp2p_m2l": (i32, [vp, i32, vp, vp, i64, vp, vp, ctypes.c_float, vp, ctypes.POINTER(dbl)]),
+        "pp2p_l2p": (i32, [vp, vp, vp]),
         "pp2p_prepare": (i32, [vp]),
         "pp2p_compute": (i32, [vp, i32]),
         "pp2p_zero_acc": (i32, [vp]),
@@ -302,6 +305,31 @@ class Context:
         n = ctypes.c_longlong(0)
         self._chk(self._lib.pp2p_walk_tree(self._h, float(open_angle), float(cutoff_radius), ctypes.byref(n)))
         return n.value
+
+    def m2l(self, center: np.ndarray, M: np.ndarray, task_t: np.ndarray, task_s: np.ndarray, rs: float,
+            L: np.ndarray | None = None) -> tuple[np.ndarray, float]:
+        """The M2L task loop on the device: returns (L + translations, kernel
+        ms).  center ncell x 3, M and L ncell x 20 (FP32)."""
+        c = np.ascontiguousarray(center, np.float32)
+        m = np.ascontiguousarray(M, np.float32)
+        out = np.zeros_like(m) if L is None else np.array(L, np.float32, copy=True)
+        tt = np.ascontiguousarray(task_t, np.int32)
+        ts = np.ascontiguousarray(task_s, np.int32)
+        if c.ndim != 2 or c.shape[1] != 3 or m.shape != (c.shape[0], 20) or out.shape != m.shape or tt.shape != ts.shape:
+            raise ValueError("center must be n x 3, M and L n x 20, task arrays of one length")
+        ms = ctypes.c_double(0.0)
+        self._chk(self._lib.pp2p_m2l(self._h, c.shape[0], _ptr(c), _ptr(m), tt.size, _ptr(tt), _ptr(ts), float(rs),
+                                     _ptr(out), ctypes.byref(ms)))
+        return out, ms.value
+
+    def l2p(self, L: np.ndarray, center: np.ndarray):
+        """Adds the force of each leaf's local expansion at its particles to
+        the accumulator."""
+        l = np.ascontiguousarray(L, np.float32)
+        c = np.ascontiguousarray(center, np.float32)
+        if l.shape != (self.nleaf, 20) or c.shape != (self.nleaf, 3):
+            raise ValueError("L must be nleaf x 20 and center nleaf x 3")
+        self._chk(self._lib.pp2p_l2p(self._h, _ptr(l), _ptr(c)))
 
     def get_tasks(self) -> tuple[np.ndarray, np.ndarray]:
         n = ctypes.c_longlong(0)

@@ -185,19 +185,21 @@ def _host_lib():
     lib.phost_get_leaves.argtypes = [vp, vp, vp, vp, vp, vp]
     lib.phost_get_nodes.argtypes = [vp, vp, vp, vp]
     lib.phost_get_tasks.argtypes = [vp, vp, vp]
+    lib.phost_get_m2l.argtypes = [vp, vp, vp]
     lib.phost_free.argtypes = [vp]
     return lib
 
 
 def build_lists(posi: np.ndarray, tag: np.ndarray, int2pos: float, maxleaf: int, open_angle: float,
                 cutoff_radius: float, bdl: np.ndarray, bdr: np.ndarray, nthreads: int = 0,
-                walk: bool = True, with_tree: bool = False) -> dict:
+                walk: bool = True, with_tree: bool = False, with_m2l: bool = False) -> dict:
     """kd-tree + dual walk (libphotons_host.so).  Returns perm (tree order ->
     input index), leaf_ip, leaf_np, active, task_t, task_s (0-based leaf ids)
     and the host seconds of the two steps (tree_s, walk_s).  walk=False skips
     the walk (empty task arrays); with_tree adds the tree as the device walk
     reads it: node_son (nnode x 2), node_center, node_width, leaf_center,
-    leaf_width (x 3, FP32)."""
+    leaf_width (x 3, FP32).  with_m2l adds the well-separated cell pairs of the
+    walk, m2l_t / m2l_s (cells: leaf id, or nleaf + node id)."""
     import time
     lib = _host_lib()
     P = lambda a: a.ctypes.data_as(__import__("ctypes").c_void_p)  # noqa: E731
@@ -235,6 +237,10 @@ def build_lists(posi: np.ndarray, tag: np.ndarray, int2pos: float, maxleaf: int,
         else:
             lib.phost_get_leaves(h, P(ip), P(npl), P(act), None, None)
         lib.phost_get_tasks(h, P(tt), P(ts))
+        if with_m2l:
+            nm = lib.phost_count(h, 4)
+            out.update(m2l_t=np.empty(nm, np.int32), m2l_s=np.empty(nm, np.int32))
+            lib.phost_get_m2l(h, P(out["m2l_t"]), P(out["m2l_s"]))
     finally:
         lib.phost_free(h)
     out.update(tree_s=t1 - t0, walk_s=t2 - t1 if walk else 0.0)

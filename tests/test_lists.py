@@ -121,11 +121,14 @@ def _accept_np(wi, wj, dist, theta, rc):
     return np.where(((m[:, 0] + m[:, 1]) + m[:, 2]).astype(np.float64) < 1.0e-15, 0, r)
 
 
-def _walk_levels(tree, active, theta, rc):
+def _walk_levels(tree, active, theta, rc, m2l=None):
     """The walk run level by level (every frame replaced by its children in
-    call order, leaf-leaf pairs carried) instead of recursively."""
+    call order, leaf-leaf pairs carried) instead of recursively.  m2l: a list
+    that receives the well-separated pairs (acceptance 1) of each level as
+    cells (leaf id, or nleaf + node id)."""
     son, nc, nw = tree["node_son"], tree["node_center"], tree["node_width"]
     lc, lw = tree["leaf_center"], tree["leaf_width"]
+    nleaf = lc.shape[0]
     im, jm = np.zeros(1, np.int64), np.zeros(1, np.int64)
     while True:
         li, lj = im < 0, jm < 0
@@ -146,7 +149,12 @@ def _walk_levels(tree, active, theta, rc):
         cj = np.where(lj[:, None], lc[np.maximum(-jm - 1, 0)], nc[np.maximum(jm, 0)])
         wi = np.where(li[:, None], lw[np.maximum(-im - 1, 0)], nw[np.maximum(im, 0)])
         wj = np.where(lj[:, None], lw[np.maximum(-jm - 1, 0)], nw[np.maximum(jm, 0)])
-        opened = rest & (_accept_np(wi, wj, ci - cj, theta, rc) == 0)
+        acc = _accept_np(wi, wj, ci - cj, theta, rc)
+        opened = rest & (acc == 0)
+        if m2l is not None:
+            sep = rest & (acc == 1)
+            cell = lambda v: np.where(v < 0, -v - 1, nleaf + v)  # noqa: E731
+            m2l.extend(zip(cell(im[sep]).tolist(), cell(jm[sep]).tolist()))
         wsi = (wi[:, 0] + wi[:, 1]) + wi[:, 2]
         wsj = (wj[:, 0] + wj[:, 1]) + wj[:, 2]
         split_i = opened & ~li & (lj | (wsi > wsj))
@@ -209,3 +217,33 @@ def test_tree_export_and_two_step_build():
                 np.float32(f["cutoff_radius"]), f["bdl"], f["bdr"])
         a = W.build_lists(*args, walk=False)
         assert a["task_t"].size == 0 and np.array_equal(a["perm"], f["perm"])
+
+
+def test_m2l_pairs_are_the_well_separated_frames():
+    """The host walk's M2L list (walk_task_m2l's tasks: the frames with
+    acceptance 1, src/fmm.c:812-900) against the level-by-level walk's
+    acceptance-1 frames, and each pair well separated by the acceptance test."""
+    g = np.random.default_rng(7)
+    box, n1, maxleaf, theta = 100.0, 20, 16, 0.5
+    pos = g.random((n1 ** 3, 3)) * box
+    pos[: pos.shape[0] // 3] = np.mod(50.0 + g.normal(0, 3.0, (pos.shape[0] // 3, 3)), box)
+    cr = 4.5 * 1.25 * box / n1
+    int2pos = W.int_grid(box, cr)
+    posi, box_i = W.to_grid(pos, box, int2pos)
+    gp, _ = W.ghost_images(posi, box_i, int(np.ceil(np.float32(cr) / int2pos)))
+    allp = np.concatenate([posi, gp])
+    tag = np.concatenate([np.ones(len(posi), np.int32), np.zeros(len(gp), np.int32)])
+    bdl, bdr = W.combine_bounds(allp, int2pos, box)
+    L = W.build_lists(allp, tag, int2pos, maxleaf, np.float32(theta), np.float32(cr), bdl, bdr, with_tree=True,
+                      with_m2l=True)
+    mine = list(zip(L["m2l_t"].tolist(), L["m2l_s"].tolist()))
+    assert len(mine) > 1000 and len(set(mine)) == len(mine)
+    ref = []
+    _walk_levels(L, L["active"], theta, cr, m2l=ref)
+    assert sorted(mine) == sorted(ref)
+    nleaf = L["leaf_ip"].size
+    cen = np.concatenate([L["leaf_center"], L["node_center"]])
+    wid = np.concatenate([L["leaf_width"], L["node_width"]])
+    t, s_ = L["m2l_t"], L["m2l_s"]
+    assert np.all(_accept_np(wid[t], wid[s_], cen[t] - cen[s_], theta, cr) == 1)
+    assert np.all((t >= nleaf) | (s_ >= nleaf))          # never two leaves
