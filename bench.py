@@ -420,6 +420,53 @@ def main():
         log(f"[bench] one step from grid positions: all on the device {dev_step_ms:.1f} ms; host tree + walk, "
             f"then upload + prepare + force {1e3 * (ls['tree'] + ls['walk']) + up_step_ms:.1f} ms "
             f"({up_step_ms:.1f} ms after the host lists)")
+    m2l = None
+    if "m2l" in w.meta and args.active <= 0 and not args.no_device_walk:
+        # the short-range M2L task loop on the device over the walk's
+        # well-separated pairs (cells: leaves, then nodes); synthetic
+        # multipoles of realistic shape (the cost does not depend on them)
+        m = w.meta
+        mt, mst = m["m2l"]
+        tree = m["tree"]
+        center = np.concatenate([tree["leaf_center"], tree["node_center"]]).astype(np.float32)
+        ncell = center.shape[0]
+        g = np.random.default_rng(20261211)
+        scale = np.array([1.0] + [0.3] * 3 + [0.1] * 6 + [0.03] * 10, np.float32)
+        Mm = (g.normal(size=(ncell, 20)).astype(np.float32) * scale)
+        Mm[:, 0] = np.abs(Mm[:, 0]) + 0.5
+        kms, walls = [], []
+        for _ in range(3):
+            t1 = time.perf_counter()
+            Lm, k_ms = ctx.m2l(center, Mm, mt, mst, w.rs)
+            walls.append(time.perf_counter() - t1)
+            kms.append(k_ms)
+        # accuracy on a sample of target cells against the FP64 oracle
+        import pyoracle
+        rows = np.unique(mt)
+        pick = g.choice(rows, size=min(300, rows.size), replace=False)
+        sel = np.isin(mt, pick)
+        ref = pyoracle.m2l_tasks_f64(center.astype(np.float64), Mm.astype(np.float64), mt[sel], mst[sel], w.rs)
+        worst = 0.0
+        for lo, hi in ((0, 1), (1, 4), (4, 10), (10, 20)):
+            sc = np.abs(ref[pick, lo:hi]).max(axis=1)
+            ok = sc > 0
+            worst = max(worst, float((np.abs(Lm[pick][ok, lo:hi] - ref[pick][ok, lo:hi]).max(axis=1)
+                                      / sc[ok]).max(initial=0.0)))
+        # CPU: the reference's task loop in its FP32 arithmetic (the oracle's
+        # restatement, one thread, the whole list), best of 2
+        cts = []
+        for _ in range(2):
+            t1 = time.perf_counter()
+            pyoracle.m2l_tasks_ref32(center, Mm, mt, mst, np.float32(w.rs))
+            cts.append(time.perf_counter() - t1)
+        cpu_rate = mt.size / min(cts)
+        m2l = {"ntask": int(mt.size), "ncell": int(ncell), "nrows": int(rows.size), "kernel_ms": min(kms),
+               "call_ms": 1e3 * min(walls), "tasks_per_s": mt.size / (min(kms) * 1e-3),
+               "worst_rel_err": worst, "cpu_ref32_tasks_per_s": cpu_rate,
+               "cpu_sample": f"all {mt.size} tasks, the reference's m2l task loop in FP32 (C restatement), 1 thread"}
+        log(f"[bench] m2l: {mt.size} well-separated pairs on {rows.size} target cells: kernel {min(kms):.3f} ms "
+            f"({mt.size / (min(kms) * 1e-3) / 1e9:.2f} G tasks/s), call {1e3 * min(walls):.1f} ms, "
+            f"worst per-order error {worst:.1e}; CPU (reference arithmetic, 1 thread) {cpu_rate / 1e6:.2f} M tasks/s")
     ctx.close()
 
     copy_cal = None
@@ -464,6 +511,7 @@ def main():
         "active_sweep": sweep,
         "copy_calibration": copy_cal,
         "lists": lists,
+        "m2l": m2l,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS,
                      "traffic": MEASURED_TRAFFIC.get(w.name, {}).get("bytes") if args.active <= 0 else None,

@@ -273,3 +273,17 @@ void oracle_m2l_tasks_f64(int ncell, const double *center, const double *M, long
         oracle_m2l_f64(dx, dy, dz, M + (size_t)ORACLE_NMULTI * s, rs, L + (size_t)ORACLE_NMULTI * t);
     }
 }
+
+/* The reference's task loop (src/fmm.c:1188-1217) in its FP32 arithmetic:
+ * L[t] += m2l(center[t] - center[s], M[s]) task by task. */
+void oracle_m2l_tasks_ref32(int ncell, const float *center, const float *M, long long ntask, const int *task_t,
+                            const int *task_s, float rs, float *L)
+{
+    (void)ncell;
+    for (long long k = 0; k < ntask; ++k) {
+        const int t = task_t[k], s = task_s[k];
+        oracle_m2l_ref32(center[3 * t] - center[3 * s], center[3 * t + 1] - center[3 * s + 1],
+                         center[3 * t + 2] - center[3 * s + 2], M + (size_t)ORACLE_NMULTI * s, rs,
+                         L + (size_t)ORACLE_NMULTI * t);
+    }
+}

@@ -60,7 +60,8 @@ def load() -> ctypes.CDLL:
                        ("oracle_m2l_f64", [dbl, dbl, dbl, vp, dbl, vp]),
                        ("oracle_l2p_f64", [dbl, dbl, dbl, vp, vp]),
                        ("oracle_p2m_f64", [i32, vp, dbl, vp, vp]),
-                       ("oracle_m2l_tasks_f64", [i32, vp, vp, i64, vp, vp, dbl, vp])):
+                       ("oracle_m2l_tasks_f64", [i32, vp, vp, i64, vp, vp, dbl, vp]),
+                       ("oracle_m2l_tasks_ref32", [i32, vp, vp, i64, vp, vp, flt, vp])):
         fn = getattr(lib, name)
         fn.restype = None
         fn.argtypes = args
@@ -263,4 +264,15 @@ def m2l_tasks_f64(center, M, task_t, task_s, rs, L0=None):
     tt = np.ascontiguousarray(task_t, np.int32)
     ts = np.ascontiguousarray(task_s, np.int32)
     load().oracle_m2l_tasks_f64(center.shape[0], _p(center), _p(M), tt.size, _p(tt), _p(ts), float(rs), _p(L))
+    return L
+
+
+def m2l_tasks_ref32(center, M, task_t, task_s, rs, L0=None):
+    """The reference's task loop in its FP32 arithmetic (the CPU baseline)."""
+    center = np.ascontiguousarray(center, np.float32)
+    M = np.ascontiguousarray(M, np.float32)
+    L = np.zeros_like(M) if L0 is None else np.array(L0, np.float32, copy=True)
+    tt = np.ascontiguousarray(task_t, np.int32)
+    ts = np.ascontiguousarray(task_s, np.int32)
+    load().oracle_m2l_tasks_ref32(center.shape[0], _p(center), _p(M), tt.size, _p(tt), _p(ts), float(rs), _p(L))
     return L

@@ -272,7 +272,7 @@ def tree_workload(name: str, pos: np.ndarray, box: float, mesh: int, maxleaf: in
     if progress:
         progress(f"{allp.shape[0]} particles ({gpos.shape[0]} ghost copies) on the grid; building tree + lists")
     lists = build_lists(allp, tag, int2pos, maxleaf, np.float32(open_angle), np.float32(cutoff_radius),
-                        bdl, bdr, nthreads, with_tree=True)
+                        bdl, bdr, nthreads, with_tree=True, with_m2l=True)
     perm = lists["perm"]
     m = dict(meta or {})
     m.update({"int2pos": int2pos, "box_i": box_i, "posi": allp[perm], "tag": tag[perm], "perm": perm,
@@ -280,6 +280,7 @@ def tree_workload(name: str, pos: np.ndarray, box: float, mesh: int, maxleaf: in
               "open_angle": open_angle, "cutoff_radius": cutoff_radius, "mesh": mesh,
               "bdl": bdl, "bdr": bdr, "active_leaf": lists["active"],
               "tree": {k: lists[k] for k in ("node_son", "node_center", "node_width", "leaf_center", "leaf_width")},
+              "m2l": (lists["m2l_t"], lists["m2l_s"]),
               "list_seconds": {"tree": lists["tree_s"], "walk": lists["walk_s"], "threads": nthreads}})
     w = Workload(name, allp[perm].astype(np.float64) * int2pos, lists["leaf_ip"], lists["leaf_np"],
                  lists["task_t"], lists["task_s"], 0.0, rs, eps, softening=softening, meta=m)
