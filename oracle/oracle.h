@@ -121,6 +121,11 @@ void oracle_l2p_f64(double dx, double dy, double dz, const double *L, double *F)
 void oracle_p2m_f64(int n, const double *pos, double mass, const double *center, double *M);
 void oracle_m2l_tasks_f64(int ncell, const double *center, const double *M, long long ntask, const int *task_t,
                           const int *task_s, double rs, double *L);
+void oracle_p2m_ref32(int npart, const int *posi, double int2pos, const float *center, float mass, float *M);
+void oracle_m2m_ref32(float dx, float dy, float dz, const float *M, float *tM);
+void oracle_l2l_ref32(float dx, float dy, float dz, const float *L, float *toL);
+void oracle_m2m_f64(double sx, double sy, double sz, const double *M, double *tM);
+void oracle_l2l_f64(double sx, double sy, double sz, const double *L, double *toL);
 void oracle_m2l_tasks_ref32(int ncell, const float *center, const float *M, long long ntask, const int *task_t,
                             const int *task_s, float rs, float *L);
 

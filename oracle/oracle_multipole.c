@@ -154,6 +154,114 @@ void oracle_l2p_ref32(int npart, const int *posi, double int2pos, const float *c
     }
 }
 
+/* src/operator.c:14-98: the multipole of npart particles (integer grid
+ * positions) about center, mass per particle `mass` (FP32). */
+void oracle_p2m_ref32(int npart, const int *posi, double int2pos, const float *center, float mass, float *M)
+{
+    for (int n = 0; n < ORACLE_NMULTI; ++n)
+        M[n] = 0.0f;
+    for (int n = 0; n < npart; ++n) {
+        const float d0 = (float)(int2pos * posi[3 * n]) - center[0];
+        const float d1 = (float)(int2pos * posi[3 * n + 1]) - center[1];
+        const float d2 = (float)(int2pos * posi[3 * n + 2]) - center[2];
+        const float m0 = mass, m1 = -m0, m2 = m0, m3 = -m0;
+        M[0] += m0;
+        M[X] += m1 * d0;
+        M[Y] += m1 * d1;
+        M[Z] += m1 * d2;
+        M[XX] += m2 * d0 * d0 / 2;
+        M[XY] += m2 * d0 * d1;
+        M[XZ] += m2 * d0 * d2;
+        M[YY] += m2 * d1 * d1 / 2;
+        M[YZ] += m2 * d1 * d2;
+        M[ZZ] += m2 * d2 * d2 / 2;
+        M[XXX] += m3 * d0 * d0 * d0 / 6;
+        M[XXY] += m3 * d1 * d0 * d0 / 2;
+        M[XXZ] += m3 * d2 * d0 * d0 / 2;
+        M[XYY] += m3 * d1 * d1 * d0 / 2;
+        M[XYZ] += m3 * d2 * d1 * d0;
+        M[XZZ] += m3 * d2 * d2 * d0 / 2;
+        M[YYY] += m3 * d1 * d1 * d1 / 6;
+        M[YYZ] += m3 * d1 * d2 * d1 / 2;
+        M[YZZ] += m3 * d2 * d2 * d1 / 2;
+        M[ZZZ] += m3 * d2 * d2 * d2 / 6;
+    }
+}
+
+/* src/operator.c:100-165: tM += M moved to a centre (dx, dy, dz) = new
+ * centre - old centre (FP32). */
+void oracle_m2m_ref32(float dx, float dy, float dz, const float *M, float *tM)
+{
+    float t[ORACLE_NMULTI];
+    t[0] = M[0];
+    t[X] = M[X] + M[0] * dx;
+    t[Y] = M[Y] + M[0] * dy;
+    t[Z] = M[Z] + M[0] * dz;
+    t[XX] = M[XX] + M[X] * dx + M[0] * dx * dx / 2;
+    t[XY] = M[XY] + M[X] * dy + M[Y] * dx + M[0] * dx * dy;
+    t[XZ] = M[XZ] + M[X] * dz + M[Z] * dx + M[0] * dx * dz;
+    t[YY] = M[YY] + M[Y] * dy + M[0] * dy * dy / 2;
+    t[YZ] = M[YZ] + M[Y] * dz + M[Z] * dy + M[0] * dy * dz;
+    t[ZZ] = M[ZZ] + M[Z] * dz + M[0] * dz * dz / 2;
+    t[XXX] = M[XXX] + M[XX] * dx + M[X] * dx * dx / 2 + M[0] * dx * dx * dx / 6;
+    t[YYY] = M[YYY] + M[YY] * dy + M[Y] * dy * dy / 2 + M[0] * dy * dy * dy / 6;
+    t[ZZZ] = M[ZZZ] + M[ZZ] * dz + M[Z] * dz * dz / 2 + M[0] * dz * dz * dz / 6;
+    t[XXY] = M[XXY] + M[XX] * dy + M[XY] * dx + M[X] * dx * dy + M[Y] * dx * dx / 2 + M[0] * dx * dx * dy / 2;
+    t[XXZ] = M[XXZ] + M[XX] * dz + M[XZ] * dx + M[X] * dx * dz + M[Z] * dx * dx / 2 + M[0] * dx * dx * dz / 2;
+    t[XYY] = M[XYY] + M[YY] * dx + M[XY] * dy + M[Y] * dx * dy + M[X] * dy * dy / 2 + M[0] * dx * dy * dy / 2;
+    t[XZZ] = M[XZZ] + M[ZZ] * dx + M[XZ] * dz + M[Z] * dx * dz + M[X] * dz * dz / 2 + M[0] * dx * dz * dz / 2;
+    t[YZZ] = M[YZZ] + M[ZZ] * dy + M[YZ] * dz + M[Z] * dy * dz + M[Y] * dz * dz / 2 + M[0] * dy * dz * dz / 2;
+    t[YYZ] = M[YYZ] + M[YY] * dz + M[YZ] * dy + M[Y] * dy * dz + M[Z] * dy * dy / 2 + M[0] * dy * dy * dz / 2;
+    t[XYZ] = M[XYZ] + M[XY] * dz + M[YZ] * dx + M[XZ] * dy + M[X] * dy * dz + M[Y] * dx * dz + M[Z] * dy * dx +
+             M[0] * dx * dy * dz;
+    for (int n = 0; n < ORACLE_NMULTI; ++n)
+        tM[n] += t[n];
+}
+
+/* src/operator.c:477-580: toL += L moved to a centre (dx, dy, dz) = new
+ * centre - old centre (FP32; the octupole terms of the first-order
+ * coefficients are summed in FP64 there, as the reference's 0.5 literals do). */
+void oracle_l2l_ref32(float dx, float dy, float dz, const float *L, float *toL)
+{
+    toL[0] += L[0] + L[X] * dx + L[Y] * dy + L[Z] * dz;
+    toL[X] += L[X];
+    toL[Y] += L[Y];
+    toL[Z] += L[Z];
+    toL[0] += L[XX] * dx * dx / 2 + L[XY] * dx * dy + L[XZ] * dx * dz + L[YZ] * dy * dz + L[YY] * dy * dy / 2 +
+              L[ZZ] * dz * dz / 2;
+    toL[X] += L[XX] * dx + L[XY] * dy + L[XZ] * dz;
+    toL[Y] += L[XY] * dx + L[YY] * dy + L[YZ] * dz;
+    toL[Z] += L[XZ] * dx + L[YZ] * dy + L[ZZ] * dz;
+    for (int n = XX; n <= ZZ; ++n)
+        toL[n] += L[n];
+    toL[0] += L[XXX] * dx * dx * dx / 6 + L[XXY] * dx * dx * dy / 2 + L[XXZ] * dx * dx * dz / 2 +
+              L[XYY] * dx * dy * dy / 2 + L[XYZ] * dx * dy * dz + L[XZZ] * dx * dz * dz / 2 +
+              L[YYY] * dy * dy * dy / 6 + L[YZZ] * dy * dz * dz / 2 + L[YYZ] * dy * dy * dz / 2 +
+              L[ZZZ] * dz * dz * dz / 6;
+    toL[X] += L[XXX] * dx * dx * 0.5 + L[XXY] * dx * dy + L[XXZ] * dx * dz + L[XYY] * dy * dy * 0.5 +
+              L[XYZ] * dy * dz + L[XZZ] * dz * dz * 0.5;
+    toL[Y] += L[XXY] * dx * dx * 0.5 + L[XYY] * dx * dy + L[XYZ] * dx * dz + L[YYY] * dy * dy * 0.5 +
+              L[YYZ] * dy * dz + L[YZZ] * dz * dz * 0.5;
+    toL[Z] += L[XXZ] * dx * dx * 0.5 + L[XYZ] * dx * dy + L[XZZ] * dx * dz + L[YYZ] * dy * dy * 0.5 +
+              L[YZZ] * dy * dz + L[ZZZ] * dz * dz * 0.5;
+    toL[XX] += L[XXX] * dx + L[XXY] * dy + L[XXZ] * dz;
+    toL[XY] += L[XXY] * dx + L[XYY] * dy + L[XYZ] * dz;
+    toL[XZ] += L[XXZ] * dx + L[XYZ] * dy + L[XZZ] * dz;
+    toL[YY] += L[XYY] * dx + L[YYY] * dy + L[YYZ] * dz;
+    toL[YZ] += L[XYZ] * dx + L[YYZ] * dy + L[YZZ] * dz;
+    toL[ZZ] += L[XZZ] * dx + L[YZZ] * dy + L[ZZZ] * dz;
+    toL[XXX] += L[XXX];
+    toL[XZZ] += L[XZZ];
+    toL[XYY] += L[XYY];
+    toL[XXY] += L[XXY];
+    toL[XXZ] += L[XXZ];
+    toL[YYY] += L[YYY];
+    toL[XYZ] += L[XYZ];
+    toL[YYZ] += L[YYZ];
+    toL[YZZ] += L[YZZ];
+    toL[ZZZ] += L[ZZZ];
+}
+
 /* ------------------------------------------------------------------ FP64 */
 
 /* f[k] = ((1/r) d/dr)^k phi for phi(r) = erfc(r / 2 rs) / r, k = 0..3, with
@@ -285,5 +393,55 @@ void oracle_m2l_tasks_ref32(int ncell, const float *center, const float *M, long
         oracle_m2l_ref32(center[3 * t] - center[3 * s], center[3 * t + 1] - center[3 * s + 1],
                          center[3 * t + 2] - center[3 * s + 2], M + (size_t)ORACLE_NMULTI * s, rs,
                          L + (size_t)ORACLE_NMULTI * t);
+    }
+}
+
+/* tM[a] += sum over b <= a of M[b] s^(a-b) / (a-b)!: the moments about a
+ * centre moved by s = new - old (since M[a] = sum m (-d)^a / a!). */
+void oracle_m2m_f64(double sx, double sy, double sz, const double *M, double *tM)
+{
+    const double sv[3] = {sx, sy, sz};
+    for (int a = 0; a < ORACLE_NMULTI; ++a) {
+        double acc = 0.0;
+        for (int b = 0; b < ORACLE_NMULTI; ++b) {
+            int e[3], ok = 1;
+            for (int c = 0; c < 3; ++c) {
+                e[c] = kExp[a][c] - kExp[b][c];
+                ok &= e[c] >= 0;
+            }
+            if (!ok)
+                continue;
+            double t = M[b];
+            for (int c = 0; c < 3; ++c)
+                for (int k = 0; k < e[c]; ++k)
+                    t *= sv[c] / (double)(k + 1);
+            acc += t;
+        }
+        tM[a] += acc;
+    }
+}
+
+/* toL[b] += sum over a >= b, |a| <= 3, of L[a] s^(a-b) / (a-b)!: the Taylor
+ * coefficients about a centre moved by s = new - old. */
+void oracle_l2l_f64(double sx, double sy, double sz, const double *L, double *toL)
+{
+    const double sv[3] = {sx, sy, sz};
+    for (int b = 0; b < ORACLE_NMULTI; ++b) {
+        double acc = 0.0;
+        for (int a = 0; a < ORACLE_NMULTI; ++a) {
+            int e[3], ok = 1;
+            for (int c = 0; c < 3; ++c) {
+                e[c] = kExp[a][c] - kExp[b][c];
+                ok &= e[c] >= 0;
+            }
+            if (!ok)
+                continue;
+            double t = L[a];
+            for (int c = 0; c < 3; ++c)
+                for (int k = 0; k < e[c]; ++k)
+                    t *= sv[c] / (double)(k + 1);
+            acc += t;
+        }
+        toL[b] += acc;
     }
 }

@@ -61,7 +61,12 @@ def load() -> ctypes.CDLL:
                        ("oracle_l2p_f64", [dbl, dbl, dbl, vp, vp]),
                        ("oracle_p2m_f64", [i32, vp, dbl, vp, vp]),
                        ("oracle_m2l_tasks_f64", [i32, vp, vp, i64, vp, vp, dbl, vp]),
-                       ("oracle_m2l_tasks_ref32", [i32, vp, vp, i64, vp, vp, flt, vp])):
+                       ("oracle_m2l_tasks_ref32", [i32, vp, vp, i64, vp, vp, flt, vp]),
+                       ("oracle_p2m_ref32", [i32, vp, dbl, vp, flt, vp]),
+                       ("oracle_m2m_ref32", [flt, flt, flt, vp, vp]),
+                       ("oracle_l2l_ref32", [flt, flt, flt, vp, vp]),
+                       ("oracle_m2m_f64", [dbl, dbl, dbl, vp, vp]),
+                       ("oracle_l2l_f64", [dbl, dbl, dbl, vp, vp])):
         fn = getattr(lib, name)
         fn.restype = None
         fn.argtypes = args
@@ -276,3 +281,33 @@ def m2l_tasks_ref32(center, M, task_t, task_s, rs, L0=None):
     ts = np.ascontiguousarray(task_s, np.int32)
     load().oracle_m2l_tasks_ref32(center.shape[0], _p(center), _p(M), tt.size, _p(tt), _p(ts), float(rs), _p(L))
     return L
+
+
+def p2m_ref32(posi, int2pos, center, mass):
+    posi = np.ascontiguousarray(posi, np.int32)
+    M = np.zeros(NMULTI, np.float32)
+    load().oracle_p2m_ref32(posi.shape[0], _p(posi), float(int2pos), _p(np.ascontiguousarray(center, np.float32)),
+                            float(mass), _p(M))
+    return M
+
+
+def _shift(fn, dtype, s, A, T0):
+    T = np.array(T0, dtype, copy=True)
+    getattr(load(), fn)(float(s[0]), float(s[1]), float(s[2]), _p(np.ascontiguousarray(A, dtype)), _p(T))
+    return T
+
+
+def m2m_ref32(s, M, T0):
+    return _shift("oracle_m2m_ref32", np.float32, s, M, T0)
+
+
+def l2l_ref32(s, L, T0):
+    return _shift("oracle_l2l_ref32", np.float32, s, L, T0)
+
+
+def m2m_f64(s, M, T0=None):
+    return _shift("oracle_m2m_f64", np.float64, s, M, np.zeros(NMULTI) if T0 is None else T0)
+
+
+def l2l_f64(s, L, T0=None):
+    return _shift("oracle_l2l_f64", np.float64, s, L, np.zeros(NMULTI) if T0 is None else T0)

@@ -7,6 +7,9 @@ QUADRUPOLE + OCTUPOLE as inc/typedef.h:12-13 set them):
        the erfc-modified kernel of the long/short split
   l2p  src/operator.c:203-262: acc[d][n] += force of the leaf's local
        expansion L at each particle of the leaf
+  m2m  src/operator.c:100-165: tM += M shifted by (dx, dy, dz)
+  l2l  src/operator.c:477-580: toL += L shifted by (dx, dy, dz)
+  p2m  src/operator.c:14-98: M of a leaf's particles about a centre
 
 Run in the build container (where oracle/_ref exists):
     python tests/golden/make_golden_m2l.py
@@ -102,6 +105,36 @@ def main():
         lib.l2p(l)
     out.update(l2p_posi=posi, l2p_ip=ip, l2p_np=sizes.astype(np.int32), l2p_center=centers, l2p_L=Lleaf,
                l2p_int2pos=np.float64(int2pos), l2p_acc0=acc0, l2p_acc=np.stack(accs))
+    # m2m and l2l: random shifts of random expansions, added to random targets
+    lib.m2m.argtypes = [ctypes.c_float, ctypes.c_float, ctypes.c_float, vp, vp]
+    lib.m2m.restype = None
+    lib.l2l.argtypes = [ctypes.c_float, ctypes.c_float, ctypes.c_float, vp, vp]
+    lib.l2l.restype = None
+    k = 200
+    sh = g.uniform(-3.0, 3.0, (k, 3)).astype(np.float32)
+    A = (g.normal(size=(k, NMULTI)) * scale).astype(np.float32)
+    T0 = (g.normal(size=(k, NMULTI)) * 0.1).astype(np.float32)
+    Tm, Tl = T0.copy(), T0.copy()
+    for i in range(k):
+        Ai = np.ascontiguousarray(A[i])
+        tm, tl = np.ascontiguousarray(Tm[i]), np.ascontiguousarray(Tl[i])
+        lib.m2m(float(sh[i, 0]), float(sh[i, 1]), float(sh[i, 2]), Ai.ctypes.data_as(vp), tm.ctypes.data_as(vp))
+        lib.l2l(float(sh[i, 0]), float(sh[i, 1]), float(sh[i, 2]), Ai.ctypes.data_as(vp), tl.ctypes.data_as(vp))
+        Tm[i], Tl[i] = tm, tl
+    out.update(shift_d=sh, shift_A=A, shift_T0=T0, m2m_T=Tm, l2l_T=Tl)
+
+    # p2m: the l2p leaves' particles about their centres, mass MASSPART
+    lib.p2m.argtypes = [ctypes.c_int, ctypes.c_int, vp, vp]
+    lib.p2m.restype = None
+    mass = 0.37
+    ctypes.c_float.in_dll(lib, "MASSPART").value = mass
+    Mp = np.zeros((nleaf, NMULTI), np.float32)
+    for l in range(nleaf):
+        c = np.ascontiguousarray(centers[l])
+        ml = np.zeros(NMULTI, np.float32)
+        lib.p2m(int(ip[l]), int(sizes[l]), c.ctypes.data_as(vp), ml.ctypes.data_as(vp))
+        Mp[l] = ml
+    out.update(p2m_M=Mp, p2m_mass=np.float32(mass))
     np.savez_compressed(os.path.join(HERE, "m2l_ref.npz"), **out)
     print(f"m2l: {n} cases, l2p: {nleaf} leaves / {npart} particles")
 

@@ -165,3 +165,46 @@ def test_m2l_task_loop_sums_tasks_and_skips_close_pairs():
             ref[t] = po.m2l_f64(d, M[s], 1.5, ref[t])
     np.testing.assert_array_equal(L, ref)
     assert np.all(L[2] == 0.0)
+
+
+def test_p2m_m2m_l2l_ref32_equal_reference_bitwise():
+    f = np.load(GOLDEN)
+    for i in range(f["shift_d"].shape[0]):
+        np.testing.assert_array_equal(po.m2m_ref32(f["shift_d"][i], f["shift_A"][i], f["shift_T0"][i]), f["m2m_T"][i])
+        np.testing.assert_array_equal(po.l2l_ref32(f["shift_d"][i], f["shift_A"][i], f["shift_T0"][i]), f["l2l_T"][i])
+    ip, npl = f["l2p_ip"], f["l2p_np"]
+    for l in range(ip.size):
+        M = po.p2m_ref32(f["l2p_posi"][ip[l]:ip[l] + npl[l]], f["l2p_int2pos"], f["l2p_center"][l], f["p2m_mass"])
+        np.testing.assert_array_equal(M, f["p2m_M"][l])
+
+
+def test_m2m_moves_moments_exactly():
+    """P2M about a child centre then M2M to the parent equals P2M about the
+    parent (a finite polynomial identity)."""
+    g = np.random.default_rng(21)
+    pos = g.uniform(-1, 1, (40, 3))
+    child, parent = np.array([0.2, -0.1, 0.3]), np.array([-0.5, 0.4, 0.1])
+    Mc = po.p2m_f64(pos, 0.3, child)
+    Mp = po.p2m_f64(pos, 0.3, parent)
+    np.testing.assert_allclose(po.m2m_f64(parent - child, Mc), Mp, rtol=1e-12, atol=1e-13)
+
+
+def test_l2l_recentres_the_local_polynomial_exactly():
+    """L2P at x of an expansion moved by L2L equals L2P of the original at the
+    same point (the cubic is re-expanded exactly)."""
+    g = np.random.default_rng(22)
+    L = g.normal(size=20)
+    s = np.array([0.7, -0.3, 0.45])
+    Lc = po.l2l_f64(s, L)
+    for x in g.uniform(-1, 1, (6, 3)):
+        np.testing.assert_allclose(po.l2p_f64(x - s, Lc), po.l2p_f64(x, L), rtol=1e-12, atol=1e-12)
+
+
+def test_f64_shifts_agree_with_reference_forms():
+    f = np.load(GOLDEN)
+    for i in range(f["shift_d"].shape[0]):
+        d, A = f["shift_d"][i].astype(np.float64), f["shift_A"][i].astype(np.float64)
+        for fn32, fn64 in ((po.m2m_ref32, po.m2m_f64), (po.l2l_ref32, po.l2l_f64)):
+            a = fn32(f["shift_d"][i], f["shift_A"][i], np.zeros(20, np.float32)).astype(np.float64)
+            b = fn64(d, A)
+            assert np.abs(a - b).max() <= 2e-6 * max(np.abs(b).max(), 1.0)
