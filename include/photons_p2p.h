@@ -204,6 +204,22 @@ int pp2p_m2l(pp2p_ctx *ctx, int ncell, const float *center, const float *M, long
  * leaves.  Synchronous. */
 int pp2p_l2p(pp2p_ctx *ctx, const float *L, const float *center);
 
+/*
+ * The multipole part of the short-range force on the tree of the last
+ * pp2p_build_tree, all on the device: P2M of the leaves (src/operator.c:
+ * 14-98, particle mass from the parameters), M2M up the tree (walk_m2m,
+ * :100-198), the M2L loop over the nm2l well-separated cell pairs (as
+ * pp2p_m2l; cells: leaf id, or nleaf + node id, the numbering of the host
+ * builder's phost_get_m2l), L2L down the tree (walk_l2l, :477-600) and L2P
+ * (:203-262), whose forces are added to the accumulator.  P2M, M2M and L2L
+ * use the reference's FP32 arithmetic operation for operation.
+ * *device_ms (nullable) receives the device time of the passes.  Synchronous.
+ */
+int pp2p_fmm_short(pp2p_ctx *ctx, long long nm2l, const int *m2l_t, const int *m2l_s, double *device_ms);
+/* The multipoles M and local expansions L (cells x 20 each, nullable) of the
+ * last pp2p_fmm_short.  Synchronous. */
+int pp2p_get_expansions(pp2p_ctx *ctx, float *M, float *L);
+
 /* Builds the target CSR and the cost-sorted work queue of the current list
  * on the device.  pp2p_compute calls it when the list changed. */
 int pp2p_prepare(pp2p_ctx *ctx);

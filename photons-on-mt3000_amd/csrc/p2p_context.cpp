@@ -156,10 +156,12 @@ struct pp2p_ctx {
     DevBuf<int> tx0, tx1, tx2, tidx, tsegof, tsmall, tsx, ttbl, ttag, tposi;
     DevBuf<int> tnodef, tleaff, tnoff, tloff;
     bool tree_ready = false;    // walk arrays, tidx and rng hold the last pp2p_build_tree
+    int tree_levels = 0;        // levels of that tree (tl[0 .. tree_levels))
 
     // M2L task loop and L2P: cells, their expansions, the task CSR
     DevBuf<float4> mcenter;
     DevBuf<float> mM, mL;
+    bool fmm_ready = false;     // mM / mL hold the expansions of the device tree's cells
     DevBuf<int> mtt, mts, mkey, msrc, mflag, mrowidx, mrow_tgt, mrow_off;
     hipEvent_t ev_m0 = nullptr, ev_m1 = nullptr;
 
@@ -659,6 +661,7 @@ int pp2p_walk_tasks(pp2p_ctx *c, int nnode, const int *node_son, const float *no
         return st;
     c->walk_nnode = nnode;
     c->tree_ready = false;
+    c->fmm_ready = false;
     return run_walk(c, open_angle, cutoff_radius, ntask);
 }
 
@@ -778,6 +781,8 @@ int pp2p_build_tree(pp2p_ctx *c, long long n, const int *posi, const int *tag, d
         return st;
     c->walk_nnode = nnode;
     c->tree_ready = true;
+    c->tree_levels = last + 1;
+    c->fmm_ready = false;
     if (nleaf_out)
         *nleaf_out = nleaf;
     if (nnode_out)
@@ -851,6 +856,9 @@ static int upload_centers(pp2p_ctx *c, int n, const float *xyz)
     return upload_boxes(c, c->mcenter, n, xyz);
 }
 
+static int m2l_core(pp2p_ctx *c, int ncell, long long ntask, const int *task_t, const int *task_s, float rs,
+                    double *ms_out);
+
 int pp2p_m2l(pp2p_ctx *c, int ncell, const float *center, const float *M, long long ntask, const int *task_t,
              const int *task_s, float rs, float *L, double *kernel_ms)
 {
@@ -874,6 +882,24 @@ int pp2p_m2l(pp2p_ctx *c, int ncell, const float *center, const float *M, long l
     HIPCHK(c->mL.reserve(nco));
     HIPCHK(hipMemcpyAsync(c->mM.p, M, sizeof(float) * nco, hipMemcpyHostToDevice, sm));
     HIPCHK(hipMemcpyAsync(c->mL.p, L, sizeof(float) * nco, hipMemcpyHostToDevice, sm));
+    c->fmm_ready = false;
+    double ms = 0.0;
+    if ((st = m2l_core(c, ncell, ntask, task_t, task_s, rs, &ms)))
+        return st;
+    HIPCHK(hipMemcpyAsync(L, c->mL.p, sizeof(float) * nco, hipMemcpyDeviceToHost, sm));
+    HIPCHK(hipStreamSynchronize(sm));
+    if (kernel_ms)
+        *kernel_ms = ms;
+    return PP2P_OK;
+}
+
+// The M2L loop over host task arrays on the cells in c->mcenter / c->mM,
+// accumulating into c->mL; *ms: the kernel's device time.
+static int m2l_core(pp2p_ctx *c, int ncell, long long ntask, const int *task_t, const int *task_s, float rs,
+                    double *ms_out)
+{
+    int st;
+    const hipStream_t sm = c->stream;
     const int n = (int)ntask;
     int nrows = 0;
     if (n > 0) {
@@ -905,13 +931,10 @@ int pp2p_m2l(pp2p_ctx *c, int ncell, const float *center, const float *M, long l
     HIPCHK(hipEventRecord(c->ev_m0, sm));
     HIPCHK(launch_m2l(a, sm));
     HIPCHK(hipEventRecord(c->ev_m1, sm));
-    HIPCHK(hipMemcpyAsync(L, c->mL.p, sizeof(float) * nco, hipMemcpyDeviceToHost, sm));
-    HIPCHK(hipStreamSynchronize(sm));
-    if (kernel_ms) {
-        float ms = 0.f;
-        HIPCHK(hipEventElapsedTime(&ms, c->ev_m0, c->ev_m1));
-        *kernel_ms = ms;
-    }
+    HIPCHK(hipEventSynchronize(c->ev_m1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev_m0, c->ev_m1));
+    *ms_out = ms;
     return PP2P_OK;
 }
 
@@ -931,8 +954,91 @@ int pp2p_l2p(pp2p_ctx *c, const float *L, const float *center)
     const size_t nco = (size_t)20 * c->nleaf;
     HIPCHK(c->mL.reserve(nco));
     HIPCHK(hipMemcpyAsync(c->mL.p, L, sizeof(float) * nco, hipMemcpyHostToDevice, c->stream));
+    c->fmm_ready = false;
     const L2PArgs a{c->rng.p, c->mcenter.p, c->mL.p, c->x3.p, c->nleaf, c->npart, c->acc.p};
     HIPCHK(launch_l2p(a, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return PP2P_OK;
+}
+
+int pp2p_fmm_short(pp2p_ctx *c, long long nm2l, const int *m2l_t, const int *m2l_s, double *device_ms)
+{
+    int st = check_ctx(c);
+    if (st)
+        return st;
+    if (!c->tree_ready)
+        return fail(PP2P_ERR_STATE, "build_tree before fmm_short");
+    if (!c->have_params)
+        return fail(PP2P_ERR_STATE, "set_params before fmm_short");
+    const int nleaf = c->nleaf, nnode = c->walk_nnode;
+    const long long ncell = (long long)nleaf + nnode;
+    if (ncell * 20 > (1ll << 31) - 1)
+        return fail(PP2P_ERR_ARG, "too many cells");
+    if (nm2l < 0 || (nm2l > 0 && (!m2l_t || !m2l_s)))
+        return fail(PP2P_ERR_ARG, "invalid m2l task arrays");
+    for (long long k = 0; k < nm2l; ++k)
+        if (m2l_t[k] < 0 || m2l_t[k] >= ncell || m2l_s[k] < 0 || m2l_s[k] >= ncell)
+            return fail(PP2P_ERR_ARG, "m2l task " + std::to_string(k) + " names a cell outside [0, ncell)");
+    const hipStream_t sm = c->stream;
+    HIPCHK(c->mcenter.reserve(ncell));
+    HIPCHK(c->mM.reserve(20 * ncell));
+    HIPCHK(c->mL.reserve(20 * ncell));
+    if (!c->ev_m0) {
+        HIPCHK(hipEventCreate(&c->ev_m0));
+        HIPCHK(hipEventCreate(&c->ev_m1));
+    }
+    const FmmTree t{nleaf, nnode, c->wson.p, c->wnc.p, c->wlc.p, c->rng.p, c->x3.p};
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, sm));
+    // upward: P2M of the leaves, M2M level by level from the deepest
+    HIPCHK(launch_cell_centers(t, c->mcenter.p, sm));
+    HIPCHK(launch_p2m(t, (float)c->params.mass, c->mM.p, sm));
+    for (int l = c->tree_levels - 1; l >= 0; --l)
+        HIPCHK(launch_m2m_level(t, c->tl[l].idb.p, c->tl[l].nseg, c->mM.p, sm));
+    HIPCHK(launch_zero_f32(c->mL.p, 20 * ncell, sm));
+    HIPCHK(hipEventRecord(e1, sm));
+    HIPCHK(hipEventSynchronize(e1));
+    float up_ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&up_ms, e0, e1));
+    // M2L over the well-separated pairs
+    double m2l_ms = 0.0;
+    if ((st = m2l_core(c, (int)ncell, nm2l, m2l_t, m2l_s, (float)c->params.rs, &m2l_ms))) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        return st;
+    }
+    // downward: L2L level by level from the root, then L2P into the accumulator
+    HIPCHK(hipEventRecord(e0, sm));
+    for (int l = 0; l < c->tree_levels; ++l)
+        HIPCHK(launch_l2l_level(t, c->tl[l].idb.p, c->tl[l].nseg, c->mL.p, sm));
+    const L2PArgs a{c->rng.p, c->wlc.p, c->mL.p, c->x3.p, nleaf, c->npart, c->acc.p};
+    HIPCHK(launch_l2p(a, sm));
+    HIPCHK(hipEventRecord(e1, sm));
+    HIPCHK(hipEventSynchronize(e1));
+    float down_ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&down_ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    c->fmm_ready = true;
+    if (device_ms)
+        *device_ms = up_ms + m2l_ms + down_ms;
+    return PP2P_OK;
+}
+
+int pp2p_get_expansions(pp2p_ctx *c, float *M, float *L)
+{
+    int st = check_ctx(c);
+    if (st)
+        return st;
+    if (!c->fmm_ready)
+        return fail(PP2P_ERR_STATE, "no fmm_short since the last tree or m2l call");
+    const size_t n = (size_t)20 * ((size_t)c->nleaf + c->walk_nnode);
+    if (M)
+        HIPCHK(hipMemcpyAsync(M, c->mM.p, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+    if (L)
+        HIPCHK(hipMemcpyAsync(L, c->mL.p, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return PP2P_OK;
 }

@@ -28,6 +28,22 @@ struct L2PArgs {
     double *acc;
 };
 
+// The device tree as the upward and downward passes read it.  Cells: leaf l
+// is cell l, node k is cell nleaf + k.
+struct FmmTree {
+    int nleaf, nnode;
+    const int2 *son;                       // node sons: >= 0 node, < 0 leaf -(id + 1)
+    const float4 *node_center, *leaf_center;
+    const int2 *rng;                       // leaf particle ranges
+    const double *x3;                      // FP64 positions, AoS
+};
+
+hipError_t launch_p2m(const FmmTree &t, float mass, float *M, hipStream_t st);
+// one level's nodes (idb[s].x = node id): M2M from their sons / L2L into them
+hipError_t launch_m2m_level(const FmmTree &t, const int2 *idb, int nseg, float *M, hipStream_t st);
+hipError_t launch_l2l_level(const FmmTree &t, const int2 *idb, int nseg, float *L, hipStream_t st);
+hipError_t launch_cell_centers(const FmmTree &t, float4 *center, hipStream_t st);
+hipError_t launch_zero_f32(float *p, long long n, hipStream_t st);
 hipError_t launch_m2l(const M2LArgs &a, hipStream_t st);
 hipError_t launch_l2p(const L2PArgs &a, hipStream_t st);
 

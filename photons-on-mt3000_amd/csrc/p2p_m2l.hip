@@ -152,6 +152,199 @@ __global__ __launch_bounds__(kM2LBlock) void k_l2p(L2PArgs a)
     }
 }
 
+// ----------------------------------------------------------- upward/downward
+// P2M (src/operator.c:14-98), M2M (:100-165, walk_m2m :171-198) and L2L
+// (:477-580, walk_l2l :582-600) in the reference's FP32 arithmetic, operation
+// for operation (no contraction), so that the multipoles and the shifted
+// local expansions equal the reference's.  Cells: leaf l is cell l, node k
+// is cell nleaf + k.
+
+__device__ void p2m_cell(const double *x3, int2 r, float4 c, float mass, float *M)
+{
+    for (int n = 0; n < 20; ++n)
+        M[n] = 0.0f;
+    for (int k = 0; k < r.y; ++k) {
+        const long long i = (long long)r.x + k;
+        const float d0 = (float)x3[3 * i] - c.x;
+        const float d1 = (float)x3[3 * i + 1] - c.y;
+        const float d2 = (float)x3[3 * i + 2] - c.z;
+        const float m0 = mass, m1 = -m0, m2 = m0, m3 = -m0;
+        M[0] += m0;
+        M[X] += m1 * d0;
+        M[Y] += m1 * d1;
+        M[Z] += m1 * d2;
+        M[XX] += m2 * d0 * d0 / 2;
+        M[XY] += m2 * d0 * d1;
+        M[XZ] += m2 * d0 * d2;
+        M[YY] += m2 * d1 * d1 / 2;
+        M[YZ] += m2 * d1 * d2;
+        M[ZZ] += m2 * d2 * d2 / 2;
+        M[XXX] += m3 * d0 * d0 * d0 / 6;
+        M[XXY] += m3 * d1 * d0 * d0 / 2;
+        M[XXZ] += m3 * d2 * d0 * d0 / 2;
+        M[XYY] += m3 * d1 * d1 * d0 / 2;
+        M[XYZ] += m3 * d2 * d1 * d0;
+        M[XZZ] += m3 * d2 * d2 * d0 / 2;
+        M[YYY] += m3 * d1 * d1 * d1 / 6;
+        M[YYZ] += m3 * d1 * d2 * d1 / 2;
+        M[YZZ] += m3 * d2 * d2 * d1 / 2;
+        M[ZZZ] += m3 * d2 * d2 * d2 / 6;
+    }
+}
+
+__device__ void m2m_add(float dx, float dy, float dz, const float *M, float *tM)
+{
+    float t[20];
+    t[0] = M[0];
+    t[X] = M[X] + M[0] * dx;
+    t[Y] = M[Y] + M[0] * dy;
+    t[Z] = M[Z] + M[0] * dz;
+    t[XX] = M[XX] + M[X] * dx + M[0] * dx * dx / 2;
+    t[XY] = M[XY] + M[X] * dy + M[Y] * dx + M[0] * dx * dy;
+    t[XZ] = M[XZ] + M[X] * dz + M[Z] * dx + M[0] * dx * dz;
+    t[YY] = M[YY] + M[Y] * dy + M[0] * dy * dy / 2;
+    t[YZ] = M[YZ] + M[Y] * dz + M[Z] * dy + M[0] * dy * dz;
+    t[ZZ] = M[ZZ] + M[Z] * dz + M[0] * dz * dz / 2;
+    t[XXX] = M[XXX] + M[XX] * dx + M[X] * dx * dx / 2 + M[0] * dx * dx * dx / 6;
+    t[YYY] = M[YYY] + M[YY] * dy + M[Y] * dy * dy / 2 + M[0] * dy * dy * dy / 6;
+    t[ZZZ] = M[ZZZ] + M[ZZ] * dz + M[Z] * dz * dz / 2 + M[0] * dz * dz * dz / 6;
+    t[XXY] = M[XXY] + M[XX] * dy + M[XY] * dx + M[X] * dx * dy + M[Y] * dx * dx / 2 + M[0] * dx * dx * dy / 2;
+    t[XXZ] = M[XXZ] + M[XX] * dz + M[XZ] * dx + M[X] * dx * dz + M[Z] * dx * dx / 2 + M[0] * dx * dx * dz / 2;
+    t[XYY] = M[XYY] + M[YY] * dx + M[XY] * dy + M[Y] * dx * dy + M[X] * dy * dy / 2 + M[0] * dx * dy * dy / 2;
+    t[XZZ] = M[XZZ] + M[ZZ] * dx + M[XZ] * dz + M[Z] * dx * dz + M[X] * dz * dz / 2 + M[0] * dx * dz * dz / 2;
+    t[YZZ] = M[YZZ] + M[ZZ] * dy + M[YZ] * dz + M[Z] * dy * dz + M[Y] * dz * dz / 2 + M[0] * dy * dz * dz / 2;
+    t[YYZ] = M[YYZ] + M[YY] * dz + M[YZ] * dy + M[Y] * dy * dz + M[Z] * dy * dy / 2 + M[0] * dy * dy * dz / 2;
+    t[XYZ] = M[XYZ] + M[XY] * dz + M[YZ] * dx + M[XZ] * dy + M[X] * dy * dz + M[Y] * dx * dz + M[Z] * dy * dx +
+             M[0] * dx * dy * dz;
+    for (int n = 0; n < 20; ++n)
+        tM[n] += t[n];
+}
+
+__device__ void l2l_add(float dx, float dy, float dz, const float *L, float *toL)
+{
+    toL[0] += L[0] + L[X] * dx + L[Y] * dy + L[Z] * dz;
+    toL[X] += L[X];
+    toL[Y] += L[Y];
+    toL[Z] += L[Z];
+    toL[0] += L[XX] * dx * dx / 2 + L[XY] * dx * dy + L[XZ] * dx * dz + L[YZ] * dy * dz + L[YY] * dy * dy / 2 +
+              L[ZZ] * dz * dz / 2;
+    toL[X] += L[XX] * dx + L[XY] * dy + L[XZ] * dz;
+    toL[Y] += L[XY] * dx + L[YY] * dy + L[YZ] * dz;
+    toL[Z] += L[XZ] * dx + L[YZ] * dy + L[ZZ] * dz;
+    for (int n = XX; n <= ZZ; ++n)
+        toL[n] += L[n];
+    toL[0] += L[XXX] * dx * dx * dx / 6 + L[XXY] * dx * dx * dy / 2 + L[XXZ] * dx * dx * dz / 2 +
+              L[XYY] * dx * dy * dy / 2 + L[XYZ] * dx * dy * dz + L[XZZ] * dx * dz * dz / 2 +
+              L[YYY] * dy * dy * dy / 6 + L[YZZ] * dy * dz * dz / 2 + L[YYZ] * dy * dy * dz / 2 +
+              L[ZZZ] * dz * dz * dz / 6;
+    // the reference's 0.5 literals: these sums are FP64, rounded once
+    toL[X] = (float)((double)toL[X] + ((double)(L[XXX] * dx * dx) * 0.5 + (double)(L[XXY] * dx * dy) +
+                                       (double)(L[XXZ] * dx * dz) + (double)(L[XYY] * dy * dy) * 0.5 +
+                                       (double)(L[XYZ] * dy * dz) + (double)(L[XZZ] * dz * dz) * 0.5));
+    toL[Y] = (float)((double)toL[Y] + ((double)(L[XXY] * dx * dx) * 0.5 + (double)(L[XYY] * dx * dy) +
+                                       (double)(L[XYZ] * dx * dz) + (double)(L[YYY] * dy * dy) * 0.5 +
+                                       (double)(L[YYZ] * dy * dz) + (double)(L[YZZ] * dz * dz) * 0.5));
+    toL[Z] = (float)((double)toL[Z] + ((double)(L[XXZ] * dx * dx) * 0.5 + (double)(L[XYZ] * dx * dy) +
+                                       (double)(L[XZZ] * dx * dz) + (double)(L[YYZ] * dy * dy) * 0.5 +
+                                       (double)(L[YZZ] * dy * dz) + (double)(L[ZZZ] * dz * dz) * 0.5));
+    toL[XX] += L[XXX] * dx + L[XXY] * dy + L[XXZ] * dz;
+    toL[XY] += L[XXY] * dx + L[XYY] * dy + L[XYZ] * dz;
+    toL[XZ] += L[XXZ] * dx + L[XYZ] * dy + L[XZZ] * dz;
+    toL[YY] += L[XYY] * dx + L[YYY] * dy + L[YYZ] * dz;
+    toL[YZ] += L[XYZ] * dx + L[YYZ] * dy + L[YZZ] * dz;
+    toL[ZZ] += L[XZZ] * dx + L[YZZ] * dy + L[ZZZ] * dz;
+    for (int n = XXX; n <= ZZZ; ++n)
+        toL[n] += L[n];
+}
+
+__global__ __launch_bounds__(kM2LBlock) void k_p2m(FmmTree t, float mass, float *M)
+{
+    const int l = blockIdx.x * kM2LBlock + threadIdx.x;
+    if (l >= t.nleaf)
+        return;
+    float m[20];
+    p2m_cell(t.x3, t.rng[l], t.leaf_center[l], mass, m);
+    for (int n = 0; n < 20; ++n)
+        M[20 * (size_t)l + n] = m[n];
+}
+
+__device__ __forceinline__ float4 cell_center(const FmmTree &t, int son)
+{
+    return son >= 0 ? t.node_center[son] : t.leaf_center[-son - 1];
+}
+
+__device__ __forceinline__ size_t cell_of(const FmmTree &t, int son)
+{
+    return son >= 0 ? (size_t)t.nleaf + son : (size_t)(-son - 1);
+}
+
+// M2M of one level's nodes: M[node] = sum over its two sons, son 0 first
+__global__ __launch_bounds__(kM2LBlock) void k_m2m_level(FmmTree t, const int2 *idb, int nseg, float *M)
+{
+    const int s = blockIdx.x * kM2LBlock + threadIdx.x;
+    if (s >= nseg)
+        return;
+    const int k = idb[s].x;
+    const float4 c = t.node_center[k];
+    const int2 son = t.son[k];
+    float acc[20];
+    for (int n = 0; n < 20; ++n)
+        acc[n] = 0.0f;
+    for (int h = 0; h < 2; ++h) {
+        const int sn = h ? son.y : son.x;
+        const float4 cs = cell_center(t, sn);
+        float m[20];
+        const float *src = M + 20 * cell_of(t, sn);
+        for (int n = 0; n < 20; ++n)
+            m[n] = src[n];
+        m2m_add(c.x - cs.x, c.y - cs.y, c.z - cs.z, m, acc);
+    }
+    float *dst = M + 20 * ((size_t)t.nleaf + k);
+    for (int n = 0; n < 20; ++n)
+        dst[n] = acc[n];
+}
+
+// L2L of one level's nodes into their sons
+__global__ __launch_bounds__(kM2LBlock) void k_l2l_level(FmmTree t, const int2 *idb, int nseg, float *L)
+{
+    const int s = blockIdx.x * kM2LBlock + threadIdx.x;
+    if (s >= nseg)
+        return;
+    const int k = idb[s].x;
+    const float4 c = t.node_center[k];
+    const int2 son = t.son[k];
+    float lp[20];
+    const float *src = L + 20 * ((size_t)t.nleaf + k);
+    for (int n = 0; n < 20; ++n)
+        lp[n] = src[n];
+    for (int h = 0; h < 2; ++h) {
+        const int sn = h ? son.y : son.x;
+        const float4 cs = cell_center(t, sn);
+        float *dst = L + 20 * cell_of(t, sn);
+        float ls[20];
+        for (int n = 0; n < 20; ++n)
+            ls[n] = dst[n];
+        l2l_add(cs.x - c.x, cs.y - c.y, cs.z - c.z, lp, ls);
+        for (int n = 0; n < 20; ++n)
+            dst[n] = ls[n];
+    }
+}
+
+__global__ __launch_bounds__(kM2LBlock) void k_cell_centers(FmmTree t, float4 *center)
+{
+    const int c = blockIdx.x * kM2LBlock + threadIdx.x;
+    if (c >= t.nleaf + t.nnode)
+        return;
+    center[c] = c < t.nleaf ? t.leaf_center[c] : t.node_center[c - t.nleaf];
+}
+
+__global__ __launch_bounds__(kM2LBlock) void k_fill_zero(float *p, long long n)
+{
+    const long long i = (long long)blockIdx.x * kM2LBlock + threadIdx.x;
+    if (i < n)
+        p[i] = 0.0f;
+}
+
 }  // namespace
 
 hipError_t launch_m2l(const M2LArgs &a, hipStream_t st)
@@ -165,6 +358,46 @@ hipError_t launch_l2p(const L2PArgs &a, hipStream_t st)
 {
     if (a.nleaf > 0)
         k_l2p<<<(unsigned)((a.nleaf + kM2LBlock / 64 - 1) / (kM2LBlock / 64)), kM2LBlock, 0, st>>>(a);
+    return hipGetLastError();
+}
+
+}  // namespace pp2p
+
+namespace pp2p {
+
+static unsigned blocks(long long n) { return (unsigned)((n + kM2LBlock - 1) / kM2LBlock); }
+
+hipError_t launch_p2m(const FmmTree &t, float mass, float *M, hipStream_t st)
+{
+    if (t.nleaf > 0)
+        k_p2m<<<blocks(t.nleaf), kM2LBlock, 0, st>>>(t, mass, M);
+    return hipGetLastError();
+}
+
+hipError_t launch_m2m_level(const FmmTree &t, const int2 *idb, int nseg, float *M, hipStream_t st)
+{
+    if (nseg > 0)
+        k_m2m_level<<<blocks(nseg), kM2LBlock, 0, st>>>(t, idb, nseg, M);
+    return hipGetLastError();
+}
+
+hipError_t launch_l2l_level(const FmmTree &t, const int2 *idb, int nseg, float *L, hipStream_t st)
+{
+    if (nseg > 0)
+        k_l2l_level<<<blocks(nseg), kM2LBlock, 0, st>>>(t, idb, nseg, L);
+    return hipGetLastError();
+}
+
+hipError_t launch_cell_centers(const FmmTree &t, float4 *center, hipStream_t st)
+{
+    k_cell_centers<<<blocks((long long)t.nleaf + t.nnode), kM2LBlock, 0, st>>>(t, center);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_f32(float *p, long long n, hipStream_t st)
+{
+    if (n > 0)
+        k_fill_zero<<<blocks(n), kM2LBlock, 0, st>>>(p, n);
     return hipGetLastError();
 }
 
