@@ -460,13 +460,30 @@ def main():
             pyoracle.m2l_tasks_ref32(center, Mm, mt, mst, np.float32(w.rs))
             cts.append(time.perf_counter() - t1)
         cpu_rate = mt.size / min(cts)
+        # the whole multipole pass on the device tree: P2M, M2M, M2L, L2L, L2P
+        perm = m["perm"]
+        posi_in = np.empty_like(m["posi"])
+        tag_in = np.empty_like(m["tag"])
+        posi_in[perm] = m["posi"]
+        tag_in[perm] = m["tag"]
+        ctx.build_tree(posi_in, tag_in, m["int2pos"], m["maxleaf"], m["bdl"], m["bdr"])
+        fmm_ms = []
+        for _ in range(3):
+            ctx.zero_acc()
+            t1 = time.perf_counter()
+            dev_ms = ctx.fmm_short(mt, mst)
+            fmm_ms.append((dev_ms, 1e3 * (time.perf_counter() - t1)))
+        fmm_dev, fmm_wall = min(fmm_ms)
         m2l = {"ntask": int(mt.size), "ncell": int(ncell), "nrows": int(rows.size), "kernel_ms": min(kms),
+               "fmm_short_device_ms": fmm_dev, "fmm_short_call_ms": fmm_wall,
                "call_ms": 1e3 * min(walls), "tasks_per_s": mt.size / (min(kms) * 1e-3),
                "worst_rel_err": worst, "cpu_ref32_tasks_per_s": cpu_rate,
                "cpu_sample": f"all {mt.size} tasks, the reference's m2l task loop in FP32 (C restatement), 1 thread"}
         log(f"[bench] m2l: {mt.size} well-separated pairs on {rows.size} target cells: kernel {min(kms):.3f} ms "
             f"({mt.size / (min(kms) * 1e-3) / 1e9:.2f} G tasks/s), call {1e3 * min(walls):.1f} ms, "
             f"worst per-order error {worst:.1e}; CPU (reference arithmetic, 1 thread) {cpu_rate / 1e6:.2f} M tasks/s")
+        log(f"[bench] multipole pass on the device tree (P2M, M2M, M2L, L2L, L2P): {fmm_dev:.2f} ms device, "
+            f"{fmm_wall:.1f} ms call")
     ctx.close()
 
     copy_cal = None
