@@ -210,12 +210,18 @@ int pp2p_l2p(pp2p_ctx *ctx, const float *L, const float *center);
  * 14-98, particle mass from the parameters), M2M up the tree (walk_m2m,
  * :100-198), the M2L loop over the nm2l well-separated cell pairs (as
  * pp2p_m2l; cells: leaf id, or nleaf + node id, the numbering of the host
- * builder's phost_get_m2l), L2L down the tree (walk_l2l, :477-600) and L2P
+ * builder's phost_get_m2l; nm2l = -1 and null arrays: the pairs of the last
+ * pp2p_walk_tree), L2L down the tree (walk_l2l, :477-600) and L2P
  * (:203-262), whose forces are added to the accumulator.  P2M, M2M and L2L
  * use the reference's FP32 arithmetic operation for operation.
  * *device_ms (nullable) receives the device time of the passes.  Synchronous.
  */
 int pp2p_fmm_short(pp2p_ctx *ctx, long long nm2l, const int *m2l_t, const int *m2l_s, double *device_ms);
+/* The well-separated cell pairs of the last device walk (pp2p_walk_tasks or
+ * pp2p_walk_tree), round by round, as cells (leaf id, or nleaf + node id);
+ * pp2p_fmm_short(ctx, -1, NULL, NULL, ...) uses them directly.  Each pointer
+ * nullable.  Synchronous. */
+int pp2p_get_m2l_tasks(pp2p_ctx *ctx, long long *n, int *m2l_t, int *m2l_s);
 /* The multipoles M and local expansions L (cells x 20 each, nullable) of the
  * last pp2p_fmm_short.  Synchronous. */
 int pp2p_get_expansions(pp2p_ctx *ctx, float *M, float *L);

@@ -138,7 +138,11 @@ struct pp2p_ctx {
     DevBuf<int> wact;
     std::vector<WalkRound> wr;
     DevBuf<int> wcnt, wpos_a, wpos_b;
-    DevBuf<unsigned long long> wemit;
+    DevBuf<unsigned long long> wemit, wsep;
+    // the walk's well-separated pairs as cells (leaf id, or nleaf + node id)
+    DevBuf<int2> wm2l_f;
+    DevBuf<int> wm2l_t, wm2l_s, wm2l_cnt;
+    long long walk_m2l_n = -1;      // -1: none since the last walk
     int walk_nnode = -1;        // nodes of the tree in the walk arrays (-1: none)
 
     // device kd-tree (pp2p_build_tree): per level its segments and sons, and
@@ -247,6 +251,7 @@ void pp2p_destroy(pp2p_ctx *c)
         r.f.release(); r.code.release(); r.off.release(); r.x.release();
     }
     c->wcnt.release(); c->wpos_a.release(); c->wpos_b.release(); c->wemit.release();
+    c->wsep.release(); c->wm2l_f.release(); c->wm2l_t.release(); c->wm2l_s.release(); c->wm2l_cnt.release();
     for (auto &l : c->tl) {
         l.seg.release(); l.kid.release(); l.leaf.release(); l.below.release(); l.idb.release(); l.split.release();
         l.mean.release(); l.np0.release(); l.kk.release(); l.lo.release(); l.hi.release();
@@ -544,8 +549,11 @@ static int run_walk(pp2p_ctx *c, float open_angle, float cutoff_radius, long lon
     const int2 root = nnode > 0 ? make_int2(0, 0) : make_int2(-1, -1);
     HIPCHK(hipMemcpyAsync(wr[0].f.p, &root, sizeof(int2), hipMemcpyHostToDevice, c->stream));
     wr[0].n = 1;
-    // per round: frames that emit a task (the list length is their sum)
+    // per round: frames that emit a task (the list length is their sum) and
+    // well-separated frames (the M2L pairs)
     HIPCHK(c->wemit.reserve(2 * (size_t)nnode + 4));
+    HIPCHK(c->wsep.reserve(2 * (size_t)nnode + 4));
+    c->walk_m2l_n = -1;
     int last = 0;
     for (int r = 0;; ++r) {
         if (r > 2 * nnode + 2)
@@ -555,13 +563,14 @@ static int run_walk(pp2p_ctx *c, float open_angle, float cutoff_radius, long lon
         if (n > (1ll << 29))
             return fail(PP2P_ERR_ARG, "tree walk round larger than 2^29 frames");
         size_t tb = 0;
-        HIPCHK(walk_code(t, wr[r].f.p, n, nullptr, nullptr, nullptr, nullptr, &tb, c->stream));
+        HIPCHK(walk_code(t, wr[r].f.p, n, nullptr, nullptr, nullptr, nullptr, nullptr, &tb, c->stream));
         if ((st = ensure_temp(c, tb)))
             return st;
         tb = c->temp.cap;
         HIPCHK(wr[r].code.reserve(n));
         HIPCHK(wr[r].off.reserve(n + 1));
-        HIPCHK(walk_code(t, wr[r].f.p, n, wr[r].code.p, wr[r].off.p, c->wemit.p + r, c->temp.p, &tb, c->stream));
+        HIPCHK(walk_code(t, wr[r].f.p, n, wr[r].code.p, wr[r].off.p, c->wemit.p + r, c->wsep.p + r, c->temp.p, &tb,
+                         c->stream));
         unsigned nn = 0;
         HIPCHK(hipMemcpyAsync(&nn, wr[r].off.p + n, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
@@ -575,13 +584,46 @@ static int run_walk(pp2p_ctx *c, float open_angle, float cutoff_radius, long lon
         HIPCHK(walk_children(t, wr[r].f.p, n, wr[r].off.p, wr[r + 1].f.p, c->stream));
         wr[r + 1].n = nn;
     }
-    std::vector<unsigned long long> em(last + 1);
+    std::vector<unsigned long long> em(last + 1), sep(last + 1);
     HIPCHK(hipMemcpyAsync(em.data(), c->wemit.p, sizeof(unsigned long long) * (last + 1), hipMemcpyDeviceToHost,
                           c->stream));
+    HIPCHK(hipMemcpyAsync(sep.data(), c->wsep.p, sizeof(unsigned long long) * (last + 1), hipMemcpyDeviceToHost,
+                          c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    unsigned long long emitted = 0;
-    for (unsigned long long v : em)
-        emitted += v;
+    unsigned long long emitted = 0, nsep = 0;
+    for (int r = 0; r <= last; ++r) {
+        emitted += em[r];
+        nsep += sep[r];
+    }
+    // the well-separated pairs, round by round in frame order, as cells
+    if (nsep > (1ull << 31) - 1)
+        return fail(PP2P_ERR_ARG, "more than 2^31-1 well-separated pairs");
+    {
+        size_t tsel = 0;
+        for (int r = 0; r <= last; ++r) {
+            size_t tb = 0;
+            HIPCHK(walk_select_m2l(nullptr, nullptr, wr[r].n, nullptr, nullptr, nullptr, &tb, c->stream));
+            tsel = std::max(tsel, tb);
+        }
+        if ((st = ensure_temp(c, tsel)))
+            return st;
+        HIPCHK(c->wm2l_f.reserve(std::max<unsigned long long>(nsep, 1)));
+        HIPCHK(c->wm2l_t.reserve(std::max<unsigned long long>(nsep, 1)));
+        HIPCHK(c->wm2l_s.reserve(std::max<unsigned long long>(nsep, 1)));
+        HIPCHK(c->wm2l_cnt.reserve(1));
+        unsigned long long o = 0;
+        for (int r = 0; r <= last; ++r) {
+            if (!sep[r])
+                continue;
+            size_t tb = c->temp.cap;
+            HIPCHK(walk_select_m2l(wr[r].f.p, wr[r].code.p, wr[r].n, c->wm2l_f.p + o, c->wm2l_cnt.p, c->temp.p, &tb,
+                                   c->stream));
+            o += sep[r];
+        }
+        HIPCHK(walk_cells(c->wm2l_f.p, (long long)nsep, c->nleaf, c->wm2l_t.p, c->wm2l_s.p, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        c->walk_m2l_n = (long long)nsep;
+    }
     if (emitted > (1ull << 31) - 1)
         return fail(PP2P_ERR_ARG, "more than 2^31-1 tasks in one list");
     // backward: tasks at and below each frame, last round first (every
@@ -783,6 +825,7 @@ int pp2p_build_tree(pp2p_ctx *c, long long n, const int *posi, const int *tag, d
     c->tree_ready = true;
     c->tree_levels = last + 1;
     c->fmm_ready = false;
+    c->walk_m2l_n = -1;
     if (nleaf_out)
         *nleaf_out = nleaf;
     if (nnode_out)
@@ -858,6 +901,17 @@ static int upload_centers(pp2p_ctx *c, int n, const float *xyz)
 
 static int m2l_core(pp2p_ctx *c, int ncell, long long ntask, const int *task_t, const int *task_s, float rs,
                     double *ms_out);
+// host task arrays to the device copies mtt / mts
+static int upload_m2l_tasks(pp2p_ctx *c, long long ntask, const int *task_t, const int *task_s)
+{
+    if (ntask <= 0)
+        return PP2P_OK;
+    HIPCHK(c->mtt.reserve(ntask));
+    HIPCHK(c->mts.reserve(ntask));
+    HIPCHK(hipMemcpyAsync(c->mtt.p, task_t, sizeof(int) * ntask, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->mts.p, task_s, sizeof(int) * ntask, hipMemcpyHostToDevice, c->stream));
+    return PP2P_OK;
+}
 
 int pp2p_m2l(pp2p_ctx *c, int ncell, const float *center, const float *M, long long ntask, const int *task_t,
              const int *task_s, float rs, float *L, double *kernel_ms)
@@ -884,7 +938,7 @@ int pp2p_m2l(pp2p_ctx *c, int ncell, const float *center, const float *M, long l
     HIPCHK(hipMemcpyAsync(c->mL.p, L, sizeof(float) * nco, hipMemcpyHostToDevice, sm));
     c->fmm_ready = false;
     double ms = 0.0;
-    if ((st = m2l_core(c, ncell, ntask, task_t, task_s, rs, &ms)))
+    if ((st = upload_m2l_tasks(c, ntask, task_t, task_s)) || (st = m2l_core(c, ncell, ntask, c->mtt.p, c->mts.p, rs, &ms)))
         return st;
     HIPCHK(hipMemcpyAsync(L, c->mL.p, sizeof(float) * nco, hipMemcpyDeviceToHost, sm));
     HIPCHK(hipStreamSynchronize(sm));
@@ -903,21 +957,19 @@ static int m2l_core(pp2p_ctx *c, int ncell, long long ntask, const int *task_t, 
     const int n = (int)ntask;
     int nrows = 0;
     if (n > 0) {
-        for (DevBuf<int> *b : {&c->mtt, &c->mts, &c->mkey, &c->msrc, &c->mflag, &c->mrowidx, &c->mrow_tgt})
+        for (DevBuf<int> *b : {&c->mkey, &c->msrc, &c->mflag, &c->mrowidx, &c->mrow_tgt})
             HIPCHK(b->reserve(n));
         HIPCHK(c->mrow_off.reserve(n + 1));
-        HIPCHK(hipMemcpyAsync(c->mtt.p, task_t, sizeof(int) * n, hipMemcpyHostToDevice, sm));
-        HIPCHK(hipMemcpyAsync(c->mts.p, task_s, sizeof(int) * n, hipMemcpyHostToDevice, sm));
         int bits = 1;
         while ((1ll << bits) < (long long)ncell)
             ++bits;
         size_t tb1 = 0, tb2 = 0;
-        HIPCHK(csr_sort(c->mtt.p, c->mts.p, c->mkey.p, c->msrc.p, n, bits, nullptr, &tb1, sm));
+        HIPCHK(csr_sort(task_t, task_s, c->mkey.p, c->msrc.p, n, bits, nullptr, &tb1, sm));
         HIPCHK(csr_rows(c->mkey.p, n, c->mflag.p, c->mrowidx.p, c->mrow_tgt.p, c->mrow_off.p, nullptr, &tb2, sm));
         if ((st = ensure_temp(c, std::max(tb1, tb2))))
             return st;
         tb1 = tb2 = c->temp.cap;
-        HIPCHK(csr_sort(c->mtt.p, c->mts.p, c->mkey.p, c->msrc.p, n, bits, c->temp.p, &tb1, sm));
+        HIPCHK(csr_sort(task_t, task_s, c->mkey.p, c->msrc.p, n, bits, c->temp.p, &tb1, sm));
         HIPCHK(csr_rows(c->mkey.p, n, c->mflag.p, c->mrowidx.p, c->mrow_tgt.p, c->mrow_off.p, c->temp.p, &tb2,
                         sm));
         HIPCHK(hipMemcpyAsync(&nrows, c->mrowidx.p + n - 1, sizeof(int), hipMemcpyDeviceToHost, sm));
@@ -974,7 +1026,10 @@ int pp2p_fmm_short(pp2p_ctx *c, long long nm2l, const int *m2l_t, const int *m2l
     const long long ncell = (long long)nleaf + nnode;
     if (ncell * 20 > (1ll << 31) - 1)
         return fail(PP2P_ERR_ARG, "too many cells");
-    if (nm2l < 0 || (nm2l > 0 && (!m2l_t || !m2l_s)))
+    const bool walked = nm2l < 0;
+    if (walked && (c->walk_m2l_n < 0 || m2l_t || m2l_s))
+        return fail(PP2P_ERR_STATE, "no well-separated pairs from pp2p_walk_tree on this tree");
+    if (!walked && nm2l > 0 && (!m2l_t || !m2l_s))
         return fail(PP2P_ERR_ARG, "invalid m2l task arrays");
     for (long long k = 0; k < nm2l; ++k)
         if (m2l_t[k] < 0 || m2l_t[k] >= ncell || m2l_s[k] < 0 || m2l_s[k] >= ncell)
@@ -1004,7 +1059,12 @@ int pp2p_fmm_short(pp2p_ctx *c, long long nm2l, const int *m2l_t, const int *m2l
     HIPCHK(hipEventElapsedTime(&up_ms, e0, e1));
     // M2L over the well-separated pairs
     double m2l_ms = 0.0;
-    if ((st = m2l_core(c, (int)ncell, nm2l, m2l_t, m2l_s, (float)c->params.rs, &m2l_ms))) {
+    if (!walked)
+        st = upload_m2l_tasks(c, nm2l, m2l_t, m2l_s);
+    if (!st)
+        st = walked ? m2l_core(c, (int)ncell, c->walk_m2l_n, c->wm2l_t.p, c->wm2l_s.p, (float)c->params.rs, &m2l_ms)
+                    : m2l_core(c, (int)ncell, nm2l, c->mtt.p, c->mts.p, (float)c->params.rs, &m2l_ms);
+    if (st) {
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         return st;
@@ -1039,6 +1099,23 @@ int pp2p_get_expansions(pp2p_ctx *c, float *M, float *L)
         HIPCHK(hipMemcpyAsync(M, c->mM.p, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
     if (L)
         HIPCHK(hipMemcpyAsync(L, c->mL.p, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return PP2P_OK;
+}
+
+int pp2p_get_m2l_tasks(pp2p_ctx *c, long long *n, int *m2l_t, int *m2l_s)
+{
+    int st = check_ctx(c);
+    if (st)
+        return st;
+    if (c->walk_m2l_n < 0)
+        return fail(PP2P_ERR_STATE, "no walk since the last tree");
+    if (n)
+        *n = c->walk_m2l_n;
+    if (c->walk_m2l_n && m2l_t)
+        HIPCHK(hipMemcpyAsync(m2l_t, c->wm2l_t.p, sizeof(int) * c->walk_m2l_n, hipMemcpyDeviceToHost, c->stream));
+    if (c->walk_m2l_n && m2l_s)
+        HIPCHK(hipMemcpyAsync(m2l_s, c->wm2l_s.p, sizeof(int) * c->walk_m2l_n, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return PP2P_OK;
 }

@@ -16,12 +16,15 @@ struct WalkTree {
     float open_angle, cutoff;
 };
 
-// Forward pass, round of n frames f: code[i] = children (0, 2, 4) | emit << 3,
+// Forward pass, round of n frames f: code[i] = children (0, 2, 4) | task << 3
+// | well-separated << 4,
 // off[0..n] = exclusive scan of the children (off[n] = next round's size),
-// *emitted = frames of the round that emit a task.
+// *emitted = frames of the round that emit a task, *separated = its
+// well-separated frames.
 // rocPRIM convention: temp == nullptr only queries *temp_bytes.
 hipError_t walk_code(const WalkTree &t, const int2 *f, long long n, unsigned char *code, unsigned *off,
-                     unsigned long long *emitted, void *temp, size_t *temp_bytes, hipStream_t st);
+                     unsigned long long *emitted, unsigned long long *separated, void *temp, size_t *temp_bytes,
+                     hipStream_t st);
 // The children of round f at their offsets in next.
 hipError_t walk_children(const WalkTree &t, const int2 *f, long long n, const unsigned *off, int2 *next,
                          hipStream_t st);
@@ -34,5 +37,12 @@ hipError_t walk_below(const unsigned char *code, const unsigned *off, const int 
 // the next round and writes the tasks of the emitting frames.
 hipError_t walk_emit(const int2 *f, const unsigned char *code, const unsigned *off, const int *xn, long long n,
                      const int *pos, int *pos_next, int *task_t, int *task_s, hipStream_t st);
+
+// The round's well-separated frames (code bit 4) compacted into out, their
+// number into *count (device).  rocPRIM convention for temp.
+hipError_t walk_select_m2l(const int2 *f, const unsigned char *code, long long n, int2 *out, int *count, void *temp,
+                           size_t *temp_bytes, hipStream_t st);
+// Frames as cells: leaf id, or nleaf + node id.
+hipError_t walk_cells(const int2 *f, long long n, int nleaf, int *ct, int *cs, hipStream_t st);
 
 }  // namespace pp2p

@@ -73,7 +73,8 @@ __device__ __forceinline__ int accept(const float4 wi, const float4 wj, float dx
 }
 
 // Child count (0, 2 or 4) of frame f, the children into ch[] when ch is not
-// null; *emit = 1 for a leaf-leaf frame whose target is active.
+// null; *emit = 1 for a leaf-leaf frame whose target is active, 2 for a
+// well-separated pair (acceptance 1: an M2L task of walk_task_m2l).
 __device__ __forceinline__ int expand_frame(const WalkTree &t, int2 f, int2 *ch, int *emit)
 {
     const int im = f.x, jm = f.y;
@@ -109,8 +110,11 @@ __device__ __forceinline__ int expand_frame(const WalkTree &t, int2 f, int2 *ch,
     }
     // leaf-node: accept(leaf, node); node-leaf: accept(leaf, node) on the
     // node-minus-leaf distance; the half-sum of widths is symmetric
-    if (accept(aw, bw, ac.x - bc.x, ac.y - bc.y, ac.z - bc.z, t.open_angle, t.cutoff) != 0)
+    const int acc = accept(aw, bw, ac.x - bc.x, ac.y - bc.y, ac.z - bc.z, t.open_angle, t.cutoff);
+    if (acc != 0) {
+        *emit = acc == 1 ? 2 : 0;
         return 0;
+    }
     if (ch) {
         if (im >= 0 && (jm < 0 || aw.x + aw.y + aw.z > bw.x + bw.y + bw.z)) {
             const int2 s = t.son[im];
@@ -129,12 +133,12 @@ struct KidsOf {
     __host__ __device__ unsigned operator()(unsigned char c) const { return c & 7u; }
 };
 struct EmitOf {
-    __host__ __device__ unsigned long long operator()(unsigned char c) const { return c >> 3; }
+    __host__ __device__ unsigned long long operator()(unsigned char c) const { return (c >> 3) & 1u; }
 };
 
 inline unsigned grid_of(long long n) { return (unsigned)((n + kWalkBlock - 1) / kWalkBlock); }
 
-// forward: code[i] = children | emit << 3
+// forward: code[i] = children | task << 3 | well-separated << 4
 __global__ __launch_bounds__(kWalkBlock) void k_walk_code(WalkTree t, const int2 *__restrict__ f, long long n,
                                                         unsigned char *__restrict__ code)
 {
@@ -143,7 +147,7 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_code(WalkTree t, const int2
         return;
     int emit;
     const int k = expand_frame(t, f[i], nullptr, &emit);
-    code[i] = (unsigned char)(k | emit << 3);
+    code[i] = (unsigned char)(k | emit << 3);      // emit 1: bit 3, emit 2: bit 4
 }
 
 // off[n] = off[n-1] + children of the last frame (the next round's size)
@@ -178,7 +182,7 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_below(const unsigned char *
     if (i >= n)
         return;
     const unsigned c = code[i];
-    int v = c >> 3;
+    int v = (c >> 3) & 1;
     if (c & 7u) {
         const unsigned o = off[i];
         v += xn[o + (c & 7u)] - xn[o];
@@ -206,7 +210,7 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_emit(const int2 *__restrict
         return;
     const unsigned c = code[i];
     const int p = pos[i];
-    if (c >> 3) {
+    if ((c >> 3) & 1) {
         const int2 fr = f[i];
         task_t[p] = -fr.x - 1;
         task_s[p] = -fr.y - 1;
@@ -221,19 +225,57 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_emit(const int2 *__restrict
     }
 }
 
+struct M2LOf {
+    __host__ __device__ unsigned char operator()(unsigned char c) const { return (c >> 4) & 1u; }
+};
+struct SepOf {
+    __host__ __device__ unsigned long long operator()(unsigned char c) const { return (c >> 4) & 1u; }
+};
+
+// frames -> (target cell, source cell): leaf id, or nleaf + node id
+__global__ __launch_bounds__(kWalkBlock) void k_walk_cells(const int2 *__restrict__ f, long long n, int nleaf,
+                                                         int *__restrict__ ct, int *__restrict__ cs)
+{
+    const long long i = (long long)blockIdx.x * kWalkBlock + threadIdx.x;
+    if (i >= n)
+        return;
+    const int2 v = f[i];
+    ct[i] = v.x < 0 ? -v.x - 1 : nleaf + v.x;
+    cs[i] = v.y < 0 ? -v.y - 1 : nleaf + v.y;
+}
+
 }  // namespace
 
+hipError_t walk_select_m2l(const int2 *f, const unsigned char *code, long long n, int2 *out, int *count, void *temp,
+                           size_t *temp_bytes, hipStream_t st)
+{
+    auto flags = rocprim::make_transform_iterator(code, M2LOf());
+    return rocprim::select(temp, *temp_bytes, f, flags, out, count, (size_t)n, st);
+}
+
+hipError_t walk_cells(const int2 *f, long long n, int nleaf, int *ct, int *cs, hipStream_t st)
+{
+    if (n > 0)
+        k_walk_cells<<<grid_of(n), kWalkBlock, 0, st>>>(f, n, nleaf, ct, cs);
+    return hipGetLastError();
+}
+
 hipError_t walk_code(const WalkTree &t, const int2 *f, long long n, unsigned char *code, unsigned *off,
-                     unsigned long long *emitted, void *temp, size_t *temp_bytes, hipStream_t st)
+                     unsigned long long *emitted, unsigned long long *separated, void *temp, size_t *temp_bytes,
+                     hipStream_t st)
 {
     auto kids = rocprim::make_transform_iterator(code, KidsOf());
     auto emits = rocprim::make_transform_iterator(code, EmitOf());
+    auto seps = rocprim::make_transform_iterator(code, SepOf());
     if (!temp) {
-        size_t a = 0, b = 0;
+        size_t a = 0, b = 0, c = 0;
         hipError_t e = rocprim::exclusive_scan(nullptr, a, kids, off, 0u, (size_t)n, rocprim::plus<unsigned>(), st);
         if (e == hipSuccess)
             e = rocprim::reduce(nullptr, b, emits, emitted, 0ull, (size_t)n, rocprim::plus<unsigned long long>(), st);
-        *temp_bytes = a > b ? a : b;
+        if (e == hipSuccess)
+            e = rocprim::reduce(nullptr, c, seps, separated, 0ull, (size_t)n, rocprim::plus<unsigned long long>(),
+                                st);
+        *temp_bytes = a > b ? (a > c ? a : c) : (b > c ? b : c);
         return e;
     }
     k_walk_code<<<grid_of(n), kWalkBlock, 0, st>>>(t, f, n, code);
@@ -241,6 +283,9 @@ hipError_t walk_code(const WalkTree &t, const int2 *f, long long n, unsigned cha
     if (e == hipSuccess)
         e = rocprim::reduce(temp, *temp_bytes, emits, emitted, 0ull, (size_t)n, rocprim::plus<unsigned long long>(),
                             st);
+    if (e == hipSuccess)
+        e = rocprim::reduce(temp, *temp_bytes, seps, separated, 0ull, (size_t)n,
+                            rocprim::plus<unsigned long long>(), st);
     if (e != hipSuccess)
         return e;
     k_walk_off_end<<<1, 64, 0, st>>>(off, code, n);

@@ -60,7 +60,7 @@ CORE_SYMBOLS = (
     "pp2p_set_params", "pp2p_set_tuning", "pp2p_get_tuning", "pp2p_set_particles_f64",
     "pp2p_set_particles_i32", "pp2p_set_active", "pp2p_set_tasks", "pp2p_add_tasks",
     "pp2p_walk_tasks", "pp2p_get_tasks", "pp2p_build_tree", "pp2p_walk_tree", "pp2p_get_tree",
-    "pp2p_m2l", "pp2p_l2p", "pp2p_fmm_short", "pp2p_get_expansions",
+    "pp2p_m2l", "pp2p_l2p", "pp2p_fmm_short", "pp2p_get_expansions", "pp2p_get_m2l_tasks",
     "pp2p_prepare", "pp2p_compute", "pp2p_zero_acc", "pp2p_run_graph", "pp2p_sync",
     "pp2p_get_acc_f64", "pp2p_get_acc_f32_soa", "pp2p_list_stats", "pp2p_last_kernel_ms",
     "pp2p_last_prepare_ms", "pp2p_count_inside", "pp2p_time_steps", "pp2p_law_eval",
@@ -105,6 +105,7 @@ def load(path: str | None = None) -> ctypes.CDLL:
         "pp2p_l2p": (i32, [vp, vp, vp]),
         "pp2p_fmm_short": (i32, [vp, i64, vp, vp, ctypes.POINTER(dbl)]),
         "pp2p_get_expansions": (i32, [vp, vp, vp]),
+        "pp2p_get_m2l_tasks": (i32, [vp, ctypes.POINTER(i64), vp, vp]),
         "pp2p_prepare": (i32, [vp]),
         "pp2p_compute": (i32, [vp, i32]),
         "pp2p_zero_acc": (i32, [vp]),
@@ -333,16 +334,28 @@ class Context:
             raise ValueError("L must be nleaf x 20 and center nleaf x 3")
         self._chk(self._lib.pp2p_l2p(self._h, _ptr(l), _ptr(c)))
 
-    def fmm_short(self, m2l_t: np.ndarray, m2l_s: np.ndarray) -> float:
-        """P2M, M2M, M2L over the given cell pairs, L2L and L2P on the device
-        tree; the forces are added to the accumulator.  Returns device ms."""
+    def fmm_short(self, m2l_t: np.ndarray | None = None, m2l_s: np.ndarray | None = None) -> float:
+        """P2M, M2M, M2L over the given cell pairs (None: the pairs of the last
+        walk_tree), L2L and L2P on the device tree; the forces are added to
+        the accumulator.  Returns device ms."""
+        ms = ctypes.c_double(0.0)
+        if m2l_t is None:
+            self._chk(self._lib.pp2p_fmm_short(self._h, -1, None, None, ctypes.byref(ms)))
+            return ms.value
         tt = np.ascontiguousarray(m2l_t, np.int32)
         ts = np.ascontiguousarray(m2l_s, np.int32)
         if tt.shape != ts.shape:
             raise ValueError("m2l_t and m2l_s differ in length")
-        ms = ctypes.c_double(0.0)
         self._chk(self._lib.pp2p_fmm_short(self._h, tt.size, _ptr(tt), _ptr(ts), ctypes.byref(ms)))
         return ms.value
+
+    def get_m2l_tasks(self) -> tuple[np.ndarray, np.ndarray]:
+        n = ctypes.c_longlong(0)
+        self._chk(self._lib.pp2p_get_m2l_tasks(self._h, ctypes.byref(n), None, None))
+        tt = np.empty(n.value, np.int32)
+        ts = np.empty(n.value, np.int32)
+        self._chk(self._lib.pp2p_get_m2l_tasks(self._h, ctypes.byref(n), _ptr(tt), _ptr(ts)))
+        return tt, ts
 
     def get_expansions(self) -> tuple[np.ndarray, np.ndarray]:
         n = self.nleaf + self.nnode

@@ -95,3 +95,31 @@ def test_fmm_short_needs_a_device_tree(gpu_ctx):
     gpu_ctx.set_particles(np.zeros((4, 3)), np.array([0], np.int32), np.array([4], np.int32))
     with pytest.raises(pp.P2PError):
         gpu_ctx.fmm_short(np.zeros(0, np.int32), np.zeros(0, np.int32))
+
+
+def test_device_walk_m2l_pairs_and_all_device_multipoles(gpu_ctx):
+    """The device walk records the same well-separated pairs as the host walk
+    (round by round instead of depth first), and the multipole pass over them
+    gives the host-list result up to the order of the FP64 sums."""
+    allp, tag, int2pos, maxleaf, bdl, bdr, cr, T = _case(20, 16, seed=99)
+    rs = cr / 4.5
+    gpu_ctx.set_tuning()
+    gpu_ctx.set_law(pp.ForceLaw(eps=0.05 * rs, rs=rs, mass=0.5))
+    gpu_ctx.build_tree(allp, tag, int2pos, maxleaf, bdl, bdr)
+    n = gpu_ctx.walk_tree(np.float32(0.5), np.float32(cr))
+    assert n == T["task_t"].size
+    mt, ms_ = gpu_ctx.get_m2l_tasks()
+    print(f"case: m2l pairs device {mt.size} host {T['m2l_t'].size}", flush=True)
+    assert sorted(zip(mt.tolist(), ms_.tolist())) == sorted(zip(T["m2l_t"].tolist(), T["m2l_s"].tolist()))
+    gpu_ctx.zero_acc()
+    gpu_ctx.fmm_short()                                  # the walk's own pairs
+    a_dev = gpu_ctx.acc()
+    _, L_dev = gpu_ctx.get_expansions()
+    gpu_ctx.zero_acc()
+    gpu_ctx.fmm_short(T["m2l_t"], T["m2l_s"])
+    a_host = gpu_ctx.acc()
+    _, L_host = gpu_ctx.get_expansions()
+    scale = np.abs(L_host).max(axis=0) + 1e-30
+    assert np.all(np.abs(L_dev - L_host) <= 1e-6 * scale)
+    mag = np.linalg.norm(a_host, axis=1).max()
+    assert np.abs(a_dev - a_host).max() <= 1e-6 * mag
