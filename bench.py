@@ -474,8 +474,31 @@ def main():
             dev_ms = ctx.fmm_short(mt, mst)
             fmm_ms.append((dev_ms, 1e3 * (time.perf_counter() - t1)))
         fmm_dev, fmm_wall = min(fmm_ms)
+
+        # the whole short-range step from grid positions in input order, all on
+        # the device: tree, walk (near pairs and well-separated pairs), P2P
+        # force, multipole pass over the walk's own pairs; best of 3
+        def short_step():
+            ctx.build_tree(posi_in, tag_in, m["int2pos"], m["maxleaf"], m["bdl"], m["bdr"])
+            ctx.walk_tree(np.float32(m["open_angle"]), np.float32(m["cutoff_radius"]))
+            ctx.prepare()
+            ctx.zero_acc()
+            ctx.compute(exec_form)
+            ctx.fmm_short()
+            ctx.sync()
+
+        sts = []
+        for _ in range(3):
+            t1 = time.perf_counter()
+            short_step()
+            sts.append(1e3 * (time.perf_counter() - t1))
+        dmt, dms = ctx.get_m2l_tasks()
+        same_pairs = bool(dmt.size == mt.size and np.array_equal(np.sort(dmt.astype(np.int64) << 32 | dms),
+                                                                 np.sort(mt.astype(np.int64) << 32 | mst)))
+        del dmt, dms
         m2l = {"ntask": int(mt.size), "ncell": int(ncell), "nrows": int(rows.size), "kernel_ms": min(kms),
                "fmm_short_device_ms": fmm_dev, "fmm_short_call_ms": fmm_wall,
+               "device_walk_pairs_identical": same_pairs, "short_step_all_device_ms": min(sts),
                "call_ms": 1e3 * min(walls), "tasks_per_s": mt.size / (min(kms) * 1e-3),
                "worst_rel_err": worst, "cpu_ref32_tasks_per_s": cpu_rate,
                "cpu_sample": f"all {mt.size} tasks, the reference's m2l task loop in FP32 (C restatement), 1 thread"}
@@ -484,6 +507,8 @@ def main():
             f"worst per-order error {worst:.1e}; CPU (reference arithmetic, 1 thread) {cpu_rate / 1e6:.2f} M tasks/s")
         log(f"[bench] multipole pass on the device tree (P2M, M2M, M2L, L2L, L2P): {fmm_dev:.2f} ms device, "
             f"{fmm_wall:.1f} ms call")
+        log(f"[bench] short-range step all on the device (tree, walk, P2P, multipoles): {min(sts):.1f} ms; "
+            f"device walk's well-separated pairs = host's: {same_pairs}")
     ctx.close()
 
     copy_cal = None

@@ -131,6 +131,7 @@ struct pp2p_ctx {
         DevBuf<unsigned char> code;
         DevBuf<unsigned> off;
         DevBuf<int> x;
+        DevBuf<int> bs, bso;            // well-separated frames per block, their scan
         long long n = 0;
     };
     DevBuf<int2> wson;
@@ -140,8 +141,7 @@ struct pp2p_ctx {
     DevBuf<int> wcnt, wpos_a, wpos_b;
     DevBuf<unsigned long long> wemit, wsep;
     // the walk's well-separated pairs as cells (leaf id, or nleaf + node id)
-    DevBuf<int2> wm2l_f;
-    DevBuf<int> wm2l_t, wm2l_s, wm2l_cnt;
+    DevBuf<int> wm2l_t, wm2l_s;
     long long walk_m2l_n = -1;      // -1: none since the last walk
     int walk_nnode = -1;        // nodes of the tree in the walk arrays (-1: none)
 
@@ -251,7 +251,7 @@ void pp2p_destroy(pp2p_ctx *c)
         r.f.release(); r.code.release(); r.off.release(); r.x.release();
     }
     c->wcnt.release(); c->wpos_a.release(); c->wpos_b.release(); c->wemit.release();
-    c->wsep.release(); c->wm2l_f.release(); c->wm2l_t.release(); c->wm2l_s.release(); c->wm2l_cnt.release();
+    c->wsep.release(); c->wm2l_t.release(); c->wm2l_s.release();
     for (auto &l : c->tl) {
         l.seg.release(); l.kid.release(); l.leaf.release(); l.below.release(); l.idb.release(); l.split.release();
         l.mean.release(); l.np0.release(); l.kk.release(); l.lo.release(); l.hi.release();
@@ -563,14 +563,15 @@ static int run_walk(pp2p_ctx *c, float open_angle, float cutoff_radius, long lon
         if (n > (1ll << 29))
             return fail(PP2P_ERR_ARG, "tree walk round larger than 2^29 frames");
         size_t tb = 0;
-        HIPCHK(walk_code(t, wr[r].f.p, n, nullptr, nullptr, nullptr, nullptr, nullptr, &tb, c->stream));
+        HIPCHK(walk_code(t, wr[r].f.p, n, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, &tb, c->stream));
         if ((st = ensure_temp(c, tb)))
             return st;
         tb = c->temp.cap;
         HIPCHK(wr[r].code.reserve(n));
         HIPCHK(wr[r].off.reserve(n + 1));
-        HIPCHK(walk_code(t, wr[r].f.p, n, wr[r].code.p, wr[r].off.p, c->wemit.p + r, c->wsep.p + r, c->temp.p, &tb,
-                         c->stream));
+        HIPCHK(wr[r].bs.reserve(walk_blocks(n)));
+        HIPCHK(walk_code(t, wr[r].f.p, n, wr[r].code.p, wr[r].off.p, wr[r].bs.p, c->wemit.p + r, c->wsep.p + r,
+                         c->temp.p, &tb, c->stream));
         unsigned nn = 0;
         HIPCHK(hipMemcpyAsync(&nn, wr[r].off.p + n, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
@@ -602,26 +603,24 @@ static int run_walk(pp2p_ctx *c, float open_angle, float cutoff_radius, long lon
         size_t tsel = 0;
         for (int r = 0; r <= last; ++r) {
             size_t tb = 0;
-            HIPCHK(walk_select_m2l(nullptr, nullptr, wr[r].n, nullptr, nullptr, nullptr, &tb, c->stream));
+            HIPCHK(walk_m2l(nullptr, nullptr, wr[r].n, nullptr, nullptr, 0, nullptr, nullptr, nullptr, &tb,
+                            c->stream));
             tsel = std::max(tsel, tb);
+            HIPCHK(wr[r].bso.reserve(walk_blocks(wr[r].n)));
         }
         if ((st = ensure_temp(c, tsel)))
             return st;
-        HIPCHK(c->wm2l_f.reserve(std::max<unsigned long long>(nsep, 1)));
         HIPCHK(c->wm2l_t.reserve(std::max<unsigned long long>(nsep, 1)));
         HIPCHK(c->wm2l_s.reserve(std::max<unsigned long long>(nsep, 1)));
-        HIPCHK(c->wm2l_cnt.reserve(1));
         unsigned long long o = 0;
         for (int r = 0; r <= last; ++r) {
             if (!sep[r])
                 continue;
             size_t tb = c->temp.cap;
-            HIPCHK(walk_select_m2l(wr[r].f.p, wr[r].code.p, wr[r].n, c->wm2l_f.p + o, c->wm2l_cnt.p, c->temp.p, &tb,
-                                   c->stream));
+            HIPCHK(walk_m2l(wr[r].f.p, wr[r].code.p, wr[r].n, wr[r].bs.p, wr[r].bso.p, c->nleaf, c->wm2l_t.p + o,
+                            c->wm2l_s.p + o, c->temp.p, &tb, c->stream));
             o += sep[r];
         }
-        HIPCHK(walk_cells(c->wm2l_f.p, (long long)nsep, c->nleaf, c->wm2l_t.p, c->wm2l_s.p, c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
         c->walk_m2l_n = (long long)nsep;
     }
     if (emitted > (1ull << 31) - 1)

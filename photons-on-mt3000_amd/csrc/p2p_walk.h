@@ -19,12 +19,15 @@ struct WalkTree {
 // Forward pass, round of n frames f: code[i] = children (0, 2, 4) | task << 3
 // | well-separated << 4,
 // off[0..n] = exclusive scan of the children (off[n] = next round's size),
+// bs[b] = well-separated frames in block b (walk_blocks(n) blocks),
 // *emitted = frames of the round that emit a task, *separated = its
 // well-separated frames.
 // rocPRIM convention: temp == nullptr only queries *temp_bytes.
-hipError_t walk_code(const WalkTree &t, const int2 *f, long long n, unsigned char *code, unsigned *off,
+hipError_t walk_code(const WalkTree &t, const int2 *f, long long n, unsigned char *code, unsigned *off, int *bs,
                      unsigned long long *emitted, unsigned long long *separated, void *temp, size_t *temp_bytes,
                      hipStream_t st);
+// Blocks of a round of n frames (the length of walk_code's bs).
+long long walk_blocks(long long n);
 // The children of round f at their offsets in next.
 hipError_t walk_children(const WalkTree &t, const int2 *f, long long n, const unsigned *off, int2 *next,
                          hipStream_t st);
@@ -38,11 +41,10 @@ hipError_t walk_below(const unsigned char *code, const unsigned *off, const int 
 hipError_t walk_emit(const int2 *f, const unsigned char *code, const unsigned *off, const int *xn, long long n,
                      const int *pos, int *pos_next, int *task_t, int *task_s, hipStream_t st);
 
-// The round's well-separated frames (code bit 4) compacted into out, their
-// number into *count (device).  rocPRIM convention for temp.
-hipError_t walk_select_m2l(const int2 *f, const unsigned char *code, long long n, int2 *out, int *count, void *temp,
-                           size_t *temp_bytes, hipStream_t st);
-// Frames as cells: leaf id, or nleaf + node id.
-hipError_t walk_cells(const int2 *f, long long n, int nleaf, int *ct, int *cs, hipStream_t st);
+// The round's well-separated frames (code bit 4), in frame order, written as
+// cells (leaf id, or nleaf + node id) to ct/cs: bso = exclusive scan of
+// walk_code's bs (walk_blocks(n) entries).  rocPRIM convention for temp.
+hipError_t walk_m2l(const int2 *f, const unsigned char *code, long long n, const int *bs, int *bso, int nleaf,
+                    int *ct, int *cs, void *temp, size_t *temp_bytes, hipStream_t st);
 
 }  // namespace pp2p

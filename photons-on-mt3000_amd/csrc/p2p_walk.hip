@@ -138,16 +138,20 @@ struct EmitOf {
 
 inline unsigned grid_of(long long n) { return (unsigned)((n + kWalkBlock - 1) / kWalkBlock); }
 
-// forward: code[i] = children | task << 3 | well-separated << 4
+// forward: code[i] = children | task << 3 | well-separated << 4, and the
+// block's number of well-separated frames into bs[block]
 __global__ __launch_bounds__(kWalkBlock) void k_walk_code(WalkTree t, const int2 *__restrict__ f, long long n,
-                                                        unsigned char *__restrict__ code)
+                                                        unsigned char *__restrict__ code, int *__restrict__ bs)
 {
     const long long i = (long long)blockIdx.x * kWalkBlock + threadIdx.x;
-    if (i >= n)
-        return;
-    int emit;
-    const int k = expand_frame(t, f[i], nullptr, &emit);
-    code[i] = (unsigned char)(k | emit << 3);      // emit 1: bit 3, emit 2: bit 4
+    int emit = 0;
+    if (i < n) {
+        const int k = expand_frame(t, f[i], nullptr, &emit);
+        code[i] = (unsigned char)(k | emit << 3);  // emit 1: bit 3, emit 2: bit 4
+    }
+    const int nsep = __syncthreads_count(emit == 2);
+    if (threadIdx.x == 0)
+        bs[blockIdx.x] = nsep;
 }
 
 // off[n] = off[n-1] + children of the last frame (the next round's size)
@@ -225,67 +229,72 @@ __global__ __launch_bounds__(kWalkBlock) void k_walk_emit(const int2 *__restrict
     }
 }
 
-struct M2LOf {
-    __host__ __device__ unsigned char operator()(unsigned char c) const { return (c >> 4) & 1u; }
-};
-struct SepOf {
-    __host__ __device__ unsigned long long operator()(unsigned char c) const { return (c >> 4) & 1u; }
-};
-
-// frames -> (target cell, source cell): leaf id, or nleaf + node id
-__global__ __launch_bounds__(kWalkBlock) void k_walk_cells(const int2 *__restrict__ f, long long n, int nleaf,
-                                                         int *__restrict__ ct, int *__restrict__ cs)
+// the well-separated frames of block b written as cells (leaf id, or
+// nleaf + node id) at out + bso[b] + their rank in the block
+__global__ __launch_bounds__(kWalkBlock) void k_walk_m2l(const int2 *__restrict__ f,
+                                                       const unsigned char *__restrict__ code, long long n,
+                                                       const int *__restrict__ bso, int nleaf,
+                                                       int *__restrict__ ct, int *__restrict__ cs)
 {
+    __shared__ int wave_base[kWalkBlock / 64];
     const long long i = (long long)blockIdx.x * kWalkBlock + threadIdx.x;
-    if (i >= n)
+    const bool sep = i < n && ((code[i] >> 4) & 1u);
+    const unsigned long long m = __ballot(sep);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0)
+        wave_base[wave] = __popcll(m);
+    __syncthreads();
+    if (!sep)
         return;
+    int o = bso[blockIdx.x] + __popcll(m & ((1ull << lane) - 1ull));
+    for (int w = 0; w < wave; ++w)
+        o += wave_base[w];
     const int2 v = f[i];
-    ct[i] = v.x < 0 ? -v.x - 1 : nleaf + v.x;
-    cs[i] = v.y < 0 ? -v.y - 1 : nleaf + v.y;
+    ct[o] = v.x < 0 ? -v.x - 1 : nleaf + v.x;
+    cs[o] = v.y < 0 ? -v.y - 1 : nleaf + v.y;
 }
 
 }  // namespace
 
-hipError_t walk_select_m2l(const int2 *f, const unsigned char *code, long long n, int2 *out, int *count, void *temp,
-                           size_t *temp_bytes, hipStream_t st)
-{
-    auto flags = rocprim::make_transform_iterator(code, M2LOf());
-    return rocprim::select(temp, *temp_bytes, f, flags, out, count, (size_t)n, st);
-}
+long long walk_blocks(long long n) { return grid_of(n); }
 
-hipError_t walk_cells(const int2 *f, long long n, int nleaf, int *ct, int *cs, hipStream_t st)
+hipError_t walk_m2l(const int2 *f, const unsigned char *code, long long n, const int *bs, int *bso, int nleaf,
+                    int *ct, int *cs, void *temp, size_t *temp_bytes, hipStream_t st)
 {
-    if (n > 0)
-        k_walk_cells<<<grid_of(n), kWalkBlock, 0, st>>>(f, n, nleaf, ct, cs);
+    const size_t nb = grid_of(n);
+    if (!temp)
+        return rocprim::exclusive_scan(nullptr, *temp_bytes, bs, bso, 0, nb, rocprim::plus<int>(), st);
+    hipError_t e = rocprim::exclusive_scan(temp, *temp_bytes, bs, bso, 0, nb, rocprim::plus<int>(), st);
+    if (e != hipSuccess)
+        return e;
+    k_walk_m2l<<<(unsigned)nb, kWalkBlock, 0, st>>>(f, code, n, bso, nleaf, ct, cs);
     return hipGetLastError();
 }
 
-hipError_t walk_code(const WalkTree &t, const int2 *f, long long n, unsigned char *code, unsigned *off,
+hipError_t walk_code(const WalkTree &t, const int2 *f, long long n, unsigned char *code, unsigned *off, int *bs,
                      unsigned long long *emitted, unsigned long long *separated, void *temp, size_t *temp_bytes,
                      hipStream_t st)
 {
     auto kids = rocprim::make_transform_iterator(code, KidsOf());
     auto emits = rocprim::make_transform_iterator(code, EmitOf());
-    auto seps = rocprim::make_transform_iterator(code, SepOf());
+    const size_t nb = grid_of(n);
     if (!temp) {
         size_t a = 0, b = 0, c = 0;
         hipError_t e = rocprim::exclusive_scan(nullptr, a, kids, off, 0u, (size_t)n, rocprim::plus<unsigned>(), st);
         if (e == hipSuccess)
             e = rocprim::reduce(nullptr, b, emits, emitted, 0ull, (size_t)n, rocprim::plus<unsigned long long>(), st);
         if (e == hipSuccess)
-            e = rocprim::reduce(nullptr, c, seps, separated, 0ull, (size_t)n, rocprim::plus<unsigned long long>(),
-                                st);
+            e = rocprim::reduce(nullptr, c, bs, separated, 0ull, nb, rocprim::plus<unsigned long long>(), st);
         *temp_bytes = a > b ? (a > c ? a : c) : (b > c ? b : c);
         return e;
     }
-    k_walk_code<<<grid_of(n), kWalkBlock, 0, st>>>(t, f, n, code);
+    k_walk_code<<<(unsigned)nb, kWalkBlock, 0, st>>>(t, f, n, code, bs);
     hipError_t e = rocprim::exclusive_scan(temp, *temp_bytes, kids, off, 0u, (size_t)n, rocprim::plus<unsigned>(), st);
     if (e == hipSuccess)
         e = rocprim::reduce(temp, *temp_bytes, emits, emitted, 0ull, (size_t)n, rocprim::plus<unsigned long long>(),
                             st);
     if (e == hipSuccess)
-        e = rocprim::reduce(temp, *temp_bytes, seps, separated, 0ull, (size_t)n,
-                            rocprim::plus<unsigned long long>(), st);
+        e = rocprim::reduce(temp, *temp_bytes, bs, separated, 0ull, nb, rocprim::plus<unsigned long long>(), st);
     if (e != hipSuccess)
         return e;
     k_walk_off_end<<<1, 64, 0, st>>>(off, code, n);
