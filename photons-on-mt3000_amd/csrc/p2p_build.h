@@ -9,6 +9,7 @@ struct RowArgs {
     const int *row_off;   // nrows + 1 offsets into src
     const int *src;       // source leaves
     const int2 *rng;      // leaf (ip, np)
+    const unsigned *nsx;  // src_prefix: ntask + 1 entries
     const int *act_cnt;   // per leaf active count (null: all particles)
     const int *act_off;   // per leaf offset into ilist
     int nrows;
@@ -32,6 +33,10 @@ hipError_t csr_sort(const int *key_in, const int *val_in, int *key_out, int *val
                     void *temp, size_t *temp_bytes, hipStream_t st);
 hipError_t csr_rows(const int *key_sorted, int n, int *flag, int *rowidx, int *row_tgt, int *row_off,
                     void *temp, size_t *temp_bytes, hipStream_t st);
+// nsx[k] = sum of the source sizes rng[src[j]].y over j < k, k = 0..n, modulo
+// 2^32 (differences within one CSR row are exact).  rocPRIM convention.
+hipError_t src_prefix(const int *src, const int2 *rng, long long n, unsigned *nsx, void *temp, size_t *temp_bytes,
+                      hipStream_t st);
 hipError_t rows_cost(const RowArgs &ra, unsigned long long *cost, unsigned long long *total,
                      void *temp, size_t *temp_bytes, hipStream_t st);
 hipError_t items_count(const RowArgs &ra, long long split, int *count, int *off,

@@ -164,6 +164,7 @@ __global__ __launch_bounds__(256) void k_active_emit(const unsigned char *__rest
 struct RowView {
     const int *row_tgt, *row_off, *src;
     const int2 *rng;
+    const unsigned *nsx;             // exclusive prefix of the source sizes (mod 2^32)
     const int *act_cnt, *act_off;   // null: all particles of the target
     float stage_ops;                 // > 0: most i-particles per item
     int ipl_max;                     // most i-particles per lane, 1..3
@@ -189,10 +190,8 @@ __global__ void k_row_cost(RowView v, int nrows, unsigned long long *__restrict_
         return;
     int ibeg, ni;
     row_i(v, v.row_tgt[r], ibeg, ni);
-    unsigned long long ns = 0;
-    for (int k = v.row_off[r]; k < v.row_off[r + 1]; ++k)
-        ns += (unsigned)v.rng[v.src[k]].y;
-    cost[r] = ns * (unsigned long long)ni;
+    const unsigned ns = v.nsx[v.row_off[r + 1]] - v.nsx[v.row_off[r]];
+    cost[r] = (unsigned long long)ns * (unsigned long long)ni;
 }
 
 // Shape of an item with c i-particles: L lanes per phase, G = 64 / L phases
@@ -250,7 +249,6 @@ __device__ int row_ranges(const RowView &v, int r, long long split, int tgt, int
     const int k0 = v.row_off[r], k1 = v.row_off[r + 1];
     int nr = 0, out = out0;
     int beg = k0;
-    long long nsum = 0;
     auto emit = [&](int s_beg, int s_end, long long ns) {
         for (int q = 0; q < k; ++q) {
             int cb, cs, lanes, ipl;
@@ -262,20 +260,34 @@ __device__ int row_ranges(const RowView &v, int r, long long split, int tgt, int
             ++out;
         }
     };
-    for (int kk = k0; kk < k1; ++kk) {
-        const long long ns = v.rng[v.src[kk]].y;
-        if (split > 0 && kk > beg && (nsum + ns) * wi > split) {
-            if (EMIT)
-                emit(beg, kk, nsum);
-            ++nr;
-            beg = kk;
-            nsum = 0;
+    // the cut after `beg` is the first kk > beg with (pairs of [beg, kk]) * wi
+    // > split: a binary search on the prefix of the source sizes (the
+    // predicate grows with kk), as a sequential scan over the row would find
+    const unsigned *P = v.nsx;
+    for (;;) {
+        int cut = k1;
+        if (split > 0) {
+            const unsigned pb = P[beg];
+            int lo = beg + 1, hi = k1;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if ((long long)(P[mid + 1] - pb) * wi > split)
+                    hi = mid;
+                else
+                    lo = mid + 1;
+            }
+            cut = lo;
         }
-        nsum += ns;
+        ++nr;
+        if (cut >= k1) {
+            if (EMIT)
+                emit(beg, k1, (long long)(P[k1] - P[beg]));
+            break;
+        }
+        if (EMIT)
+            emit(beg, cut, (long long)(P[cut] - P[beg]));
+        beg = cut;
     }
-    if (EMIT)
-        emit(beg, k1, nsum);
-    ++nr;
     return nr * k;
 }
 
@@ -315,6 +327,17 @@ __global__ void k_iota(int *p, int n)
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k < n)
         p[k] = k;
+}
+
+struct SrcSize {
+    const int2 *rng;
+    __host__ __device__ unsigned operator()(int s) const { return (unsigned)rng[s].y; }
+};
+
+__global__ void k_zero_u32(unsigned *p)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0)
+        p[0] = 0u;
 }
 
 // ---------------------------------------------------------------- host side
@@ -401,13 +424,26 @@ hipError_t csr_rows(const int *key_sorted, int n, int *flag, int *rowidx, int *r
     return hipGetLastError();
 }
 
+hipError_t src_prefix(const int *src, const int2 *rng, long long n, unsigned *nsx, void *temp, size_t *temp_bytes,
+                      hipStream_t st)
+{
+    auto sizes = rocprim::make_transform_iterator(src, SrcSize{rng});
+    if (temp == nullptr)
+        return rocprim::inclusive_scan(nullptr, *temp_bytes, sizes, nsx + 1, (size_t)n, rocprim::plus<unsigned>(),
+                                       st);
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, nsx);
+    if (n <= 0)
+        return hipGetLastError();
+    return rocprim::inclusive_scan(temp, *temp_bytes, sizes, nsx + 1, (size_t)n, rocprim::plus<unsigned>(), st);
+}
+
 hipError_t rows_cost(const RowArgs &ra, unsigned long long *cost, unsigned long long *total,
                      void *temp, size_t *temp_bytes, hipStream_t st)
 {
     if (temp == nullptr)
         return rocprim::reduce(nullptr, *temp_bytes, cost, total, 0ull, (size_t)ra.nrows,
                                rocprim::plus<unsigned long long>(), st);
-    RowView v{ra.row_tgt, ra.row_off, ra.src, ra.rng, ra.act_cnt, ra.act_off, ra.stage_ops, ra.ipl_max};
+    RowView v{ra.row_tgt, ra.row_off, ra.src, ra.rng, ra.nsx, ra.act_cnt, ra.act_off, ra.stage_ops, ra.ipl_max};
     hipLaunchKernelGGL(k_row_cost, dim3(grid_of(ra.nrows)), dim3(256), 0, st, v, ra.nrows, cost);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess)
@@ -422,7 +458,7 @@ hipError_t items_count(const RowArgs &ra, long long split, int *count, int *off,
     if (temp == nullptr)
         return rocprim::exclusive_scan(nullptr, *temp_bytes, count, off, 0, (size_t)ra.nrows + 1,
                                        rocprim::plus<int>(), st);
-    RowView v{ra.row_tgt, ra.row_off, ra.src, ra.rng, ra.act_cnt, ra.act_off, ra.stage_ops, ra.ipl_max};
+    RowView v{ra.row_tgt, ra.row_off, ra.src, ra.rng, ra.nsx, ra.act_cnt, ra.act_off, ra.stage_ops, ra.ipl_max};
     hipLaunchKernelGGL(k_items_count, dim3(grid_of(ra.nrows)), dim3(256), 0, st, v, ra.nrows, split, count);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess)
@@ -433,7 +469,7 @@ hipError_t items_count(const RowArgs &ra, long long split, int *count, int *off,
 
 hipError_t items_emit(const RowArgs &ra, long long split, const int *off, Item *items, unsigned *cost, hipStream_t st)
 {
-    RowView v{ra.row_tgt, ra.row_off, ra.src, ra.rng, ra.act_cnt, ra.act_off, ra.stage_ops, ra.ipl_max};
+    RowView v{ra.row_tgt, ra.row_off, ra.src, ra.rng, ra.nsx, ra.act_cnt, ra.act_off, ra.stage_ops, ra.ipl_max};
     hipLaunchKernelGGL(k_items_emit, dim3(grid_of(ra.nrows)), dim3(256), 0, st, v, ra.nrows, split, off, items, cost);
     return hipGetLastError();
 }

@@ -110,6 +110,7 @@ struct pp2p_ctx {
 
     // items
     DevBuf<unsigned long long> row_cost, total;
+    DevBuf<unsigned> nsx;               // prefix of the source sizes over the CSR
     DevBuf<int> item_count, item_off;
     DevBuf<Item> items_rows, items_q;     // row order / cost-sorted
     DevBuf<unsigned> cost, cost_sorted;
@@ -1177,6 +1178,7 @@ static RowArgs row_args(pp2p_ctx *c)
     ra.row_off = c->row_off.p;
     ra.src = c->src.p;
     ra.rng = c->rng.p;
+    ra.nsx = c->nsx.p;
     ra.act_cnt = c->has_act ? c->act_cnt.p : nullptr;
     ra.act_off = c->has_act ? c->act_off.p : nullptr;
     ra.nrows = c->nrows;
@@ -1215,7 +1217,7 @@ static int queue_groups(pp2p_ctx *c)
 // (QUEUE form).
 static int build_items(pp2p_ctx *c)
 {
-    const RowArgs ra = row_args(c);
+    RowArgs ra = row_args(c);
     const int nr = c->nrows;
     HIPCHK(c->row_cost.reserve(std::max(nr, 1)));
     HIPCHK(c->total.reserve(1));
@@ -1226,13 +1228,17 @@ static int build_items(pp2p_ctx *c)
     c->list_pairs = 0;
     if (nr == 0)
         return PP2P_OK;
-    size_t tb1 = 0, tb2 = 0;
+    HIPCHK(c->nsx.reserve(c->ntask + 1));
+    ra.nsx = c->nsx.p;
+    size_t tb0 = 0, tb1 = 0, tb2 = 0;
+    HIPCHK(src_prefix(c->src.p, c->rng.p, c->ntask, c->nsx.p, nullptr, &tb0, c->stream));
     HIPCHK(rows_cost(ra, c->row_cost.p, c->total.p, nullptr, &tb1, c->stream));
     HIPCHK(items_count(ra, 0, c->item_count.p, c->item_off.p, nullptr, &tb2, c->stream));
-    int st = ensure_temp(c, std::max(tb1, tb2));
+    int st = ensure_temp(c, std::max(tb0, std::max(tb1, tb2)));
     if (st)
         return st;
-    tb1 = tb2 = c->temp.cap;
+    tb0 = tb1 = tb2 = c->temp.cap;
+    HIPCHK(src_prefix(c->src.p, c->rng.p, c->ntask, c->nsx.p, c->temp.p, &tb0, c->stream));
     HIPCHK(rows_cost(ra, c->row_cost.p, c->total.p, c->temp.p, &tb1, c->stream));
     unsigned long long total = 0;
     HIPCHK(hipMemcpyAsync(&total, c->total.p, sizeof(total), hipMemcpyDeviceToHost, c->stream));
