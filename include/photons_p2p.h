@@ -226,6 +226,24 @@ int pp2p_get_m2l_tasks(pp2p_ctx *ctx, long long *n, int *m2l_t, int *m2l_s);
  * last pp2p_fmm_short.  Synchronous. */
 int pp2p_get_expansions(pp2p_ctx *ctx, float *M, float *L);
 
+/* Long-range particle-mesh accelerations on one GPU holding the whole
+ * periodic mesh: the force part of partmesh_kick (src/partmesh.c:1476-2954)
+ * with its convolution (src/conv.f90:168-352).  posi: npart x 3 int32 grid
+ * positions (Body.posi of INTXYZ), x = (float)(posi * int2pos); tag: NULL
+ * (every particle local) or npart ints, local iff tag == 1 as the reference
+ * tests part[n].tag.  box = BOXSIZE, nside = NSIDEMESH (even, 8..1024),
+ * split_radius = splitRadius, mass = MASSPART.  Cell-centred CIC deposit,
+ * FFT, Green's function exp(-k^2 rs^2) / k^2 with the CIC deconvolution,
+ * inverse FFT, 4-point gradient at the 8 CIC cells.  acc: 3 x npart FP32
+ * (SoA, the reference's acc[3][NPART]); zero for non-local particles.  The
+ * mesh, the transforms and the gradient are FP64 (the reference: FP32 mesh).
+ * *device_ms: deposit to interpolation, on the device.  Synchronous. */
+int pp2p_pm_forces(pp2p_ctx *ctx, long long npart, const int *posi, const int *tag, double int2pos,
+                   float box, int nside, float split_radius, float mass, float *acc, double *device_ms);
+/* The last pp2p_pm_forces call's mesh: which = 0 the deposited mass per cell,
+ * 1 the potential; nside^3 doubles, C order [x][y][z]. */
+int pp2p_pm_get_mesh(pp2p_ctx *ctx, int which, double *out);
+
 /* Builds the target CSR and the cost-sorted work queue of the current list
  * on the device.  pp2p_compute calls it when the list changed. */
 int pp2p_prepare(pp2p_ctx *ctx);

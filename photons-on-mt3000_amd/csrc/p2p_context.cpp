@@ -2,6 +2,7 @@
 // step (target CSR + work items) and the launches.  One HIP stream per
 // context; every call returns a pp2p_status and records a message on error.
 #include <hip/hip_runtime.h>
+#include <hipfft/hipfft.h>
 
 #include <algorithm>
 #include <cmath>
@@ -15,6 +16,7 @@
 #include "p2p_build.h"
 #include "p2p_launch.h"
 #include "p2p_m2l.h"
+#include "p2p_pm.h"
 #include "p2p_tree.h"
 #include "p2p_walk.h"
 
@@ -170,6 +172,15 @@ struct pp2p_ctx {
     DevBuf<int> mtt, mts, mkey, msrc, mflag, mrowidx, mrow_tgt, mrow_off;
     hipEvent_t ev_m0 = nullptr, ev_m1 = nullptr;
 
+    // particle mesh: positions, tags, mass mesh, spectrum, potential, result
+    DevBuf<int> pm_posi, pm_tag;
+    DevBuf<double> pm_rho, pm_pot;
+    DevBuf<double2> pm_spec;
+    DevBuf<float> pm_acc;
+    hipfftHandle pm_fwd = 0, pm_inv = 0;
+    int pm_nside = 0;                  // the plans' mesh size (0: none)
+    bool pm_ready = false;             // pm_rho / pm_pot hold the last call's meshes
+
     float last_kernel_ms = 0.f, last_prepare_ms = 0.f;
     bool kernel_timed = false, prepare_timed = false;
 };
@@ -249,7 +260,7 @@ void pp2p_destroy(pp2p_ctx *c)
     c->items_q.release(); c->cost.release(); c->cost_sorted.release();
     c->wson.release(); c->wnc.release(); c->wnw.release(); c->wlc.release(); c->wlw.release(); c->wact.release();
     for (auto &r : c->wr) {
-        r.f.release(); r.code.release(); r.off.release(); r.x.release();
+        r.f.release(); r.code.release(); r.off.release(); r.x.release(); r.bs.release(); r.bso.release();
     }
     c->wcnt.release(); c->wpos_a.release(); c->wpos_b.release(); c->wemit.release();
     c->wsep.release(); c->wm2l_t.release(); c->wm2l_s.release();
@@ -265,6 +276,11 @@ void pp2p_destroy(pp2p_ctx *c)
         b->release();
     if (c->ev_m0) (void)hipEventDestroy(c->ev_m0);
     if (c->ev_m1) (void)hipEventDestroy(c->ev_m1);
+    c->pm_posi.release(); c->pm_tag.release(); c->pm_rho.release(); c->pm_pot.release(); c->pm_spec.release();
+    c->pm_acc.release();
+    if (c->pm_fwd) (void)hipfftDestroy(c->pm_fwd);
+    if (c->pm_inv) (void)hipfftDestroy(c->pm_inv);
+    c->nsx.release();
     c->idx.release(); c->idx_sorted.release(); c->counter.release(); c->temp.release();
     c->pair_count.release();
     if (c->ev_k0) (void)hipEventDestroy(c->ev_k0);
@@ -1625,3 +1641,100 @@ int pp2p_law_eval(const pp2p_params *p, int segs, long long n, const float *s, f
 double pp2p_last_prepare_ms(pp2p_ctx *c) { return c ? c->last_prepare_ms : -1.0; }
 
 }  // extern "C"
+
+// ------------------------------------------------------------ particle mesh
+
+#define FFTCHK(expr)                                                                      \
+    do {                                                                                  \
+        hipfftResult r_ = (expr);                                                         \
+        if (r_ != HIPFFT_SUCCESS)                                                         \
+            return fail(PP2P_ERR_DEVICE, std::string(#expr) + ": hipFFT error " + std::to_string((int)r_)); \
+    } while (0)
+
+static int pm_plans(pp2p_ctx *c, int nside)
+{
+    if (c->pm_nside == nside)
+        return PP2P_OK;
+    if (c->pm_fwd) (void)hipfftDestroy(c->pm_fwd);
+    if (c->pm_inv) (void)hipfftDestroy(c->pm_inv);
+    c->pm_fwd = c->pm_inv = 0;
+    c->pm_nside = 0;
+    FFTCHK(hipfftPlan3d(&c->pm_fwd, nside, nside, nside, HIPFFT_D2Z));
+    FFTCHK(hipfftPlan3d(&c->pm_inv, nside, nside, nside, HIPFFT_Z2D));
+    FFTCHK(hipfftSetStream(c->pm_fwd, c->stream));
+    FFTCHK(hipfftSetStream(c->pm_inv, c->stream));
+    c->pm_nside = nside;
+    return PP2P_OK;
+}
+
+int pp2p_pm_forces(pp2p_ctx *c, long long npart, const int *posi, const int *tag, double int2pos, float box,
+                   int nside, float split_radius, float mass, float *acc, double *device_ms)
+{
+    int st = check_ctx(c);
+    if (st)
+        return st;
+    if (npart < 0 || (npart > 0 && (!posi || !acc)))
+        return fail(PP2P_ERR_ARG, "invalid particle arrays");
+    if (nside < 8 || nside > 1024 || (nside & 1))
+        return fail(PP2P_ERR_ARG, "mesh size must be even and in [8, 1024]");
+    if (!(box > 0.f) || std::isinf(box) || !(int2pos > 0.0) || std::isinf(int2pos) || !(split_radius >= 0.f) ||
+        std::isinf(split_radius) || std::isnan(mass) || std::isinf(mass))
+        return fail(PP2P_ERR_ARG, "invalid box, grid step, split radius or mass");
+    if (npart > (1ll << 31) - 1)
+        return fail(PP2P_ERR_ARG, "more than 2^31-1 particles");
+    const hipStream_t sm = c->stream;
+    const long long ncell = (long long)nside * nside * nside;
+    const long long nspec = (long long)nside * nside * (nside / 2 + 1);
+    c->pm_ready = false;
+    if ((st = pm_plans(c, nside)))
+        return st;
+    HIPCHK(c->pm_posi.reserve(std::max(3 * npart, 1ll)));
+    HIPCHK(c->pm_tag.reserve(std::max(npart, 1ll)));
+    HIPCHK(c->pm_rho.reserve(ncell));
+    HIPCHK(c->pm_pot.reserve(ncell));
+    HIPCHK(c->pm_spec.reserve(nspec));
+    HIPCHK(c->pm_acc.reserve(std::max(3 * npart, 1ll)));
+    if (!c->ev_m0) {
+        HIPCHK(hipEventCreate(&c->ev_m0));
+        HIPCHK(hipEventCreate(&c->ev_m1));
+    }
+    if (npart > 0) {
+        HIPCHK(hipMemcpyAsync(c->pm_posi.p, posi, sizeof(int) * 3 * npart, hipMemcpyHostToDevice, sm));
+        if (tag)
+            HIPCHK(hipMemcpyAsync(c->pm_tag.p, tag, sizeof(int) * npart, hipMemcpyHostToDevice, sm));
+    }
+    PmArgs a{c->pm_posi.p, tag ? c->pm_tag.p : nullptr, npart, int2pos, box, mass, nside};
+    HIPCHK(hipEventRecord(c->ev_m0, sm));
+    HIPCHK(pm_zero(c->pm_rho.p, ncell, sm));
+    HIPCHK(pm_deposit(a, c->pm_rho.p, sm));
+    FFTCHK(hipfftExecD2Z(c->pm_fwd, c->pm_rho.p, reinterpret_cast<hipfftDoubleComplex *>(c->pm_spec.p)));
+    HIPCHK(pm_green(nside, split_radius, box, c->pm_spec.p, sm));
+    FFTCHK(hipfftExecZ2D(c->pm_inv, reinterpret_cast<hipfftDoubleComplex *>(c->pm_spec.p), c->pm_pot.p));
+    HIPCHK(pm_interp(a, c->pm_pot.p, c->pm_acc.p, sm));
+    HIPCHK(hipEventRecord(c->ev_m1, sm));
+    if (npart > 0)
+        HIPCHK(hipMemcpyAsync(acc, c->pm_acc.p, sizeof(float) * 3 * npart, hipMemcpyDeviceToHost, sm));
+    HIPCHK(hipStreamSynchronize(sm));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev_m0, c->ev_m1));
+    if (device_ms)
+        *device_ms = ms;
+    c->pm_ready = true;
+    return PP2P_OK;
+}
+
+int pp2p_pm_get_mesh(pp2p_ctx *c, int which, double *out)
+{
+    int st = check_ctx(c);
+    if (st)
+        return st;
+    if (!c->pm_ready)
+        return fail(PP2P_ERR_STATE, "pm_forces before pm_get_mesh");
+    if ((which != 0 && which != 1) || !out)
+        return fail(PP2P_ERR_ARG, "which must be 0 (mass) or 1 (potential), with an output array");
+    const long long ncell = (long long)c->pm_nside * c->pm_nside * c->pm_nside;
+    HIPCHK(hipMemcpyAsync(out, which == 0 ? c->pm_rho.p : c->pm_pot.p, sizeof(double) * ncell,
+                          hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return PP2P_OK;
+}

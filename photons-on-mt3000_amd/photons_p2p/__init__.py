@@ -61,6 +61,7 @@ CORE_SYMBOLS = (
     "pp2p_set_particles_i32", "pp2p_set_active", "pp2p_set_tasks", "pp2p_add_tasks",
     "pp2p_walk_tasks", "pp2p_get_tasks", "pp2p_build_tree", "pp2p_walk_tree", "pp2p_get_tree",
     "pp2p_m2l", "pp2p_l2p", "pp2p_fmm_short", "pp2p_get_expansions", "pp2p_get_m2l_tasks",
+    "pp2p_pm_forces", "pp2p_pm_get_mesh",
     "pp2p_prepare", "pp2p_compute", "pp2p_zero_acc", "pp2p_run_graph", "pp2p_sync",
     "pp2p_get_acc_f64", "pp2p_get_acc_f32_soa", "pp2p_list_stats", "pp2p_last_kernel_ms",
     "pp2p_last_prepare_ms", "pp2p_count_inside", "pp2p_time_steps", "pp2p_law_eval",
@@ -106,6 +107,9 @@ def load(path: str | None = None) -> ctypes.CDLL:
         "pp2p_fmm_short": (i32, [vp, i64, vp, vp, ctypes.POINTER(dbl)]),
         "pp2p_get_expansions": (i32, [vp, vp, vp]),
         "pp2p_get_m2l_tasks": (i32, [vp, ctypes.POINTER(i64), vp, vp]),
+        "pp2p_pm_forces": (i32, [vp, i64, vp, vp, dbl, ctypes.c_float, i32, ctypes.c_float, ctypes.c_float, vp,
+                                 ctypes.POINTER(dbl)]),
+        "pp2p_pm_get_mesh": (i32, [vp, i32, vp]),
         "pp2p_prepare": (i32, [vp]),
         "pp2p_compute": (i32, [vp, i32]),
         "pp2p_zero_acc": (i32, [vp]),
@@ -348,6 +352,33 @@ class Context:
             raise ValueError("m2l_t and m2l_s differ in length")
         self._chk(self._lib.pp2p_fmm_short(self._h, tt.size, _ptr(tt), _ptr(ts), ctypes.byref(ms)))
         return ms.value
+
+    def pm_forces(self, posi: np.ndarray, tag: np.ndarray | None, int2pos: float, box: float, nside: int,
+                  split_radius: float, mass: float) -> tuple[np.ndarray, float]:
+        """Long-range PM accelerations (n, 3) FP32 of the int32 grid positions
+        posi (n, 3); tag None: all local, else local iff tag == 1.  Returns
+        (acc, device ms)."""
+        p = np.ascontiguousarray(posi, np.int32).reshape(-1, 3)
+        n = p.shape[0]
+        t = None if tag is None else np.ascontiguousarray(tag, np.int32)
+        if t is not None and t.size != n:
+            raise ValueError("tag and posi differ in length")
+        acc = np.empty((3, n), np.float32)
+        ms = ctypes.c_double(0.0)
+        self._chk(self._lib.pp2p_pm_forces(self._h, n, _ptr(p), None if t is None else _ptr(t), float(int2pos),
+                                           float(box), int(nside), float(split_radius), float(mass), _ptr(acc),
+                                           ctypes.byref(ms)))
+        self._pm_nside = int(nside)
+        return acc.T.copy(), ms.value
+
+    def pm_mesh(self, which: int) -> np.ndarray:
+        """The last pm_forces call's mass (0) or potential (1) mesh."""
+        nside = getattr(self, "_pm_nside", 0)
+        if nside <= 0:
+            raise P2PError("pm_forces before pm_mesh")
+        out = np.empty((nside, nside, nside), np.float64)
+        self._chk(self._lib.pp2p_pm_get_mesh(self._h, int(which), _ptr(out)))
+        return out
 
     def get_m2l_tasks(self) -> tuple[np.ndarray, np.ndarray]:
         n = ctypes.c_longlong(0)

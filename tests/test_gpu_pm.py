@@ -1,0 +1,75 @@
+"""The device particle mesh (pp2p_pm_forces) against the PM oracle
+(oracle/oracle_pm.py, pinned in test_oracle_pm.py): the deposited mass mesh,
+the potential and the accelerations on random and clustered particles, with
+and without tags, the vanishing total force at a larger size, and the
+host-side argument checks."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import photons_p2p as pp
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+import oracle_pm as op  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+BITS = 2 ** 30
+
+
+def _particles(g, n, box, clustered):
+    x = g.uniform(0, box, (n, 3))
+    if clustered:
+        k = n // 2
+        x[:k] = (0.37 * box + g.normal(size=(k, 3)) * 0.02 * box) % box
+    return np.round(x / (box / BITS)).astype(np.int64).clip(0, BITS - 1).astype(np.int32)
+
+
+@pytest.mark.parametrize("N,box,clustered,tagged", [(16, 100.0, False, False), (32, 50.0, True, False),
+                                                     (32, 400.0, True, True), (48, 100.0, False, True)])
+def test_pm_matches_the_oracle(gpu_ctx, N, box, clustered, tagged):
+    g = np.random.default_rng(N + int(box))
+    n = 3000
+    posi = _particles(g, n, box, clustered)
+    tag = (g.uniform(size=n) < 0.7).astype(np.int32) if tagged else None
+    rs, mass, int2pos = 1.25 * box / N, 0.37, box / BITS
+    print(f"case: N={N} box={box} clustered={clustered} tagged={tagged}", flush=True)
+    acc, ms = gpu_ctx.pm_forces(posi, tag, int2pos, box, N, rs, mass)
+    ref, rho_ref, pot_ref = op.pm_forces(posi, tag, int2pos, np.float32(box), N, np.float32(rs), np.float32(mass))
+    rho = gpu_ctx.pm_mesh(0)
+    pot = gpu_ctx.pm_mesh(1)
+    # the mass mesh: the same FP32 products summed in FP64 in another order
+    np.testing.assert_allclose(rho, rho_ref, rtol=1e-12, atol=1e-12 * rho_ref.max())
+    # the potential: FP64 transforms on both sides
+    assert np.abs(pot - pot_ref).max() <= 1e-10 * np.abs(pot_ref).max()
+    # accelerations: FP64 on both sides, FP32 output
+    scale = np.abs(ref).max()
+    assert np.abs(acc - ref).max() <= 2e-6 * scale
+    if tagged:
+        assert np.all(acc[tag != 1] == 0.0)
+    assert ms > 0.0
+
+
+def test_pm_total_force_vanishes_at_size(gpu_ctx):
+    """128^3 mesh, 2^20 particles, half in a clump: CIC + antisymmetric
+    gradient + symmetric Green's function conserve momentum."""
+    g = np.random.default_rng(77)
+    box, N = 200.0, 128
+    posi = _particles(g, 1 << 20, box, True)
+    acc, ms = gpu_ctx.pm_forces(posi, None, box / BITS, box, N, 1.25 * box / N, 1.0)
+    tot = acc.astype(np.float64).sum(0)
+    print(f"sum {tot}, sum |a| {np.abs(acc).sum():.3e}, {ms:.2f} ms", flush=True)
+    assert np.abs(tot).max() <= 1e-6 * np.abs(acc.astype(np.float64)).sum()
+
+
+def test_pm_rejects_bad_arguments_on_the_host(gpu_ctx):
+    posi = np.zeros((4, 3), np.int32)
+    for N in (7, 6, 2048):
+        with pytest.raises(pp.P2PError):
+            gpu_ctx.pm_forces(posi, None, 1e-6, 100.0, N, 1.0, 1.0)
+    with pytest.raises(pp.P2PError):
+        gpu_ctx.pm_forces(posi, None, 1e-6, -1.0, 16, 1.0, 1.0)
+    with pytest.raises(pp.P2PError):
+        gpu_ctx.pm_forces(posi, None, 0.0, 100.0, 16, 1.0, 1.0)
