@@ -192,6 +192,7 @@ def main():
     ap.add_argument("--compare-plain", action="store_true",
                     help="also time the one-launch-per-target form the same way and report the ratio")
     ap.add_argument("--chunk-model", type=int, default=0, help="i-chunking: 0 default, -1 chunks of 64, >0 cost")
+    ap.add_argument("--no-pm", action="store_true", help="skip the particle-mesh measurement")
     ap.add_argument("--no-device-walk", action="store_true",
                     help="skip building the tree and the list again on the device (tree workloads)")
     args = ap.parse_args()
@@ -509,6 +510,43 @@ def main():
             f"{fmm_wall:.1f} ms call")
         log(f"[bench] short-range step all on the device (tree, walk, P2P, multipoles): {min(sts):.1f} ms; "
             f"device walk's well-separated pairs = host's: {same_pairs}")
+    pm = None
+    if "box_i" in w.meta and args.active <= 0 and not args.no_pm:
+        # the long-range particle mesh of the same particles (local ones,
+        # tag 1): mesh of the split (rs = 1.25 cells), whole box on the device
+        m = w.meta
+        box = float(m["box_i"] * m["int2pos"])
+        nside = int(round(1.25 * box / w.rs))
+        nside += nside & 1
+        calls = []
+        for _ in range(3):
+            t1 = time.perf_counter()
+            acc_pm, dev_ms = ctx.pm_forces(m["posi"], m["tag"], m["int2pos"], box, nside, w.rs, w.mass)
+            calls.append((dev_ms, 1e3 * (time.perf_counter() - t1)))
+        pm_dev, pm_call = min(calls)
+        import oracle_pm
+        loc = m["tag"] == 1
+        rho = ctx.pm_mesh(0)
+        mass_err = abs(rho.sum() / (loc.sum() * float(np.float32(w.mass))) - 1.0)
+        a64 = acc_pm[loc].astype(np.float64)
+        momentum = float(np.abs(a64.sum(0)).max() / np.abs(a64).sum())
+        # the oracle's transform of the device mass mesh, and its gradient and
+        # interpolation at 20,000 sampled particles of the device potential
+        pot = ctx.pm_mesh(1)
+        pot_ref = oracle_pm.potential(rho, np.float32(w.rs), np.float32(box))
+        pot_err = float(np.abs(pot - pot_ref).max() / np.abs(pot_ref).max())
+        del pot_ref, rho
+        pick = np.random.default_rng(5).choice(np.nonzero(loc)[0], size=min(20000, int(loc.sum())), replace=False)
+        a_ref = oracle_pm.accelerations(pot, m["posi"][pick], m["int2pos"], np.float32(box))
+        acc_err = float(np.abs(acc_pm[pick] - a_ref).max() / np.abs(a_ref).max())
+        del pot
+        pm = {"nside": nside, "box": box, "nlocal": int(loc.sum()), "device_ms": pm_dev, "call_ms": pm_call,
+              "mass_rel_err": mass_err, "momentum_rel": momentum, "potential_rel_err_vs_oracle_fft": pot_err,
+              "acc_rel_err_vs_oracle_sample": acc_err}
+        log(f"[bench] pm: {int(loc.sum())} particles on a {nside}^3 mesh: {pm_dev:.2f} ms device "
+            f"(deposit, FFT, Green, inverse FFT, gradient + interpolation), {pm_call:.1f} ms per call with the "
+            f"upload; mass {mass_err:.1e}, |sum a| / sum |a| {momentum:.1e}, potential vs oracle FFT {pot_err:.1e}, "
+            f"acc vs oracle (20000 particles) {acc_err:.1e}")
     ctx.close()
 
     copy_cal = None
@@ -554,6 +592,7 @@ def main():
         "copy_calibration": copy_cal,
         "lists": lists,
         "m2l": m2l,
+        "pm": pm,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS,
                      "traffic": MEASURED_TRAFFIC.get(w.name, {}).get("bytes") if args.active <= 0 else None,

@@ -174,7 +174,7 @@ struct pp2p_ctx {
 
     // particle mesh: positions, tags, mass mesh, spectrum, potential, result
     DevBuf<int> pm_posi, pm_tag;
-    DevBuf<double> pm_rho, pm_pot;
+    DevBuf<double> pm_rho, pm_pot, pm_grad;
     DevBuf<double2> pm_spec;
     DevBuf<float> pm_acc;
     hipfftHandle pm_fwd = 0, pm_inv = 0;
@@ -276,7 +276,7 @@ void pp2p_destroy(pp2p_ctx *c)
         b->release();
     if (c->ev_m0) (void)hipEventDestroy(c->ev_m0);
     if (c->ev_m1) (void)hipEventDestroy(c->ev_m1);
-    c->pm_posi.release(); c->pm_tag.release(); c->pm_rho.release(); c->pm_pot.release(); c->pm_spec.release();
+    c->pm_posi.release(); c->pm_tag.release(); c->pm_rho.release(); c->pm_pot.release(); c->pm_grad.release(); c->pm_spec.release();
     c->pm_acc.release();
     if (c->pm_fwd) (void)hipfftDestroy(c->pm_fwd);
     if (c->pm_inv) (void)hipfftDestroy(c->pm_inv);
@@ -1692,6 +1692,7 @@ int pp2p_pm_forces(pp2p_ctx *c, long long npart, const int *posi, const int *tag
     HIPCHK(c->pm_tag.reserve(std::max(npart, 1ll)));
     HIPCHK(c->pm_rho.reserve(ncell));
     HIPCHK(c->pm_pot.reserve(ncell));
+    HIPCHK(c->pm_grad.reserve(3 * ncell));
     HIPCHK(c->pm_spec.reserve(nspec));
     HIPCHK(c->pm_acc.reserve(std::max(3 * npart, 1ll)));
     if (!c->ev_m0) {
@@ -1710,7 +1711,7 @@ int pp2p_pm_forces(pp2p_ctx *c, long long npart, const int *posi, const int *tag
     FFTCHK(hipfftExecD2Z(c->pm_fwd, c->pm_rho.p, reinterpret_cast<hipfftDoubleComplex *>(c->pm_spec.p)));
     HIPCHK(pm_green(nside, split_radius, box, c->pm_spec.p, sm));
     FFTCHK(hipfftExecZ2D(c->pm_inv, reinterpret_cast<hipfftDoubleComplex *>(c->pm_spec.p), c->pm_pot.p));
-    HIPCHK(pm_interp(a, c->pm_pot.p, c->pm_acc.p, sm));
+    HIPCHK(pm_interp(a, c->pm_pot.p, c->pm_grad.p, c->pm_acc.p, sm));
     HIPCHK(hipEventRecord(c->ev_m1, sm));
     if (npart > 0)
         HIPCHK(hipMemcpyAsync(acc, c->pm_acc.p, sizeof(float) * 3 * npart, hipMemcpyDeviceToHost, sm));

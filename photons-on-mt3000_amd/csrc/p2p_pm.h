@@ -19,9 +19,10 @@ hipError_t pm_deposit(const PmArgs &a, double *rho, hipStream_t st);
 // spec[N][N][N/2+1] *= G(k) (the long-range Green's function with the CIC
 // deconvolution; spec[0] = 0).
 hipError_t pm_green(int nside, float split_radius, float box, double2 *spec, hipStream_t st);
-// acc[3][n] (SoA) = 4-point gradient of pot[N^3] interpolated with the CIC
-// weights; zero for particles that are not local.
-hipError_t pm_interp(const PmArgs &a, const double *pot, float *acc, hipStream_t st);
+// grad[3][N^3] = 4-point gradient of pot[N^3] per cell; acc[3][n] (SoA) =
+// grad interpolated with the CIC weights; zero for particles that are not
+// local.
+hipError_t pm_interp(const PmArgs &a, const double *pot, double *grad, float *acc, hipStream_t st);
 // p[0..n) = 0
 hipError_t pm_zero(double *p, long long n, hipStream_t st);
 

@@ -5,7 +5,8 @@
 //   INT2POS); i = (int)(posf * norm); w = (posf - (i + 0.5) delta) norm with the
 //   neighbour on the side of w (src/partmesh.c:1569-1612).
 // * The deposit adds the FP32 products MASSPART * wx * wy * wz into an FP64
-//   mesh with atomics (the reference sums in FP32 in particle order).
+//   mesh with atomics, first summed per chunk of particles in LDS (the
+//   reference sums in FP32 in particle order).
 // * The Green's function multiplies the FP64 r2c spectrum in place.
 // * The acceleration: 4-point central differences of the potential at the 8
 //   CIC cells, weighted by the CIC weights, in FP64; stored as FP32 like the
@@ -59,22 +60,78 @@ __device__ __forceinline__ void particle_axes(const PmArgs &a, long long p, Axis
     }
 }
 
+// One block deposits a chunk of kPmChunk consecutive particles.  Particles
+// that arrive in spatial order (tree order) cover few cells per chunk: the
+// chunk's cells (unwrapped indices, both CIC neighbours) are bounded first,
+// and when the box fits in kPmTile cells the chunk sums into LDS and adds each
+// touched cell to the mesh once; otherwise every corner goes to the mesh.
+constexpr int kPmChunk = 1024;
+constexpr int kPmTile = 4096;
+
+__device__ __forceinline__ bool local_particle(const PmArgs &a, long long p)
+{
+    return p < a.n && (!a.tag || a.tag[p] == 1);
+}
+
 __global__ __launch_bounds__(kPmBlock) void k_pm_deposit(PmArgs a, double *__restrict__ rho)
 {
-    const long long p = (long long)blockIdx.x * kPmBlock + threadIdx.x;
-    if (p >= a.n || (a.tag && a.tag[p] != 1))
-        return;
-    Axis ax[3];
-    particle_axes(a, p, ax);
+    __shared__ double tile[kPmTile];
+    __shared__ int lo[3], hi[3];
     const int N = a.nside;
-    const int cx[2] = {wrap(ax[0].i, N), wrap(ax[0].ii, N)};
-    const int cy[2] = {wrap(ax[1].i, N), wrap(ax[1].ii, N)};
-    const int cz[2] = {wrap(ax[2].i, N), wrap(ax[2].ii, N)};
-    const float wx[2] = {ax[0].wn, ax[0].w}, wy[2] = {ax[1].wn, ax[1].w}, wz[2] = {ax[2].wn, ax[2].w};
-    for (int c = 0; c < 8; ++c) {
-        const int x = c & 1, y = (c >> 1) & 1, z = c >> 2;
-        const float m = ((a.mass * wx[x]) * wy[y]) * wz[z];
-        atomicAdd(&rho[((long long)cx[x] * N + cy[y]) * N + cz[z]], (double)m);
+    const long long p0 = (long long)blockIdx.x * kPmChunk;
+    if (threadIdx.x < 3) {
+        lo[threadIdx.x] = 0x7fffffff;
+        hi[threadIdx.x] = -0x7fffffff;
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < kPmChunk; k += kPmBlock) {
+        const long long p = p0 + k;
+        if (!local_particle(a, p))
+            continue;
+        Axis ax[3];
+        particle_axes(a, p, ax);
+        for (int d = 0; d < 3; ++d) {
+            atomicMin(&lo[d], min(ax[d].i, ax[d].ii));
+            atomicMax(&hi[d], max(ax[d].i, ax[d].ii));
+        }
+    }
+    __syncthreads();
+    if (hi[0] < lo[0])
+        return;                                   // no local particle in the chunk
+    const int sx = hi[0] - lo[0] + 1, sy = hi[1] - lo[1] + 1, sz = hi[2] - lo[2] + 1;
+    const bool fits = (long long)sx * sy * sz <= kPmTile;
+    if (fits) {
+        for (int c = threadIdx.x; c < kPmTile; c += kPmBlock)
+            tile[c] = 0.0;
+        __syncthreads();
+    }
+    for (int k = threadIdx.x; k < kPmChunk; k += kPmBlock) {
+        const long long p = p0 + k;
+        if (!local_particle(a, p))
+            continue;
+        Axis ax[3];
+        particle_axes(a, p, ax);
+        const int cx[2] = {ax[0].i, ax[0].ii}, cy[2] = {ax[1].i, ax[1].ii}, cz[2] = {ax[2].i, ax[2].ii};
+        const float wx[2] = {ax[0].wn, ax[0].w}, wy[2] = {ax[1].wn, ax[1].w}, wz[2] = {ax[2].wn, ax[2].w};
+        for (int c = 0; c < 8; ++c) {
+            const int x = c & 1, y = (c >> 1) & 1, z = c >> 2;
+            const float m = ((a.mass * wx[x]) * wy[y]) * wz[z];
+            if (fits)
+                atomicAdd(&tile[((cx[x] - lo[0]) * sy + (cy[y] - lo[1])) * sz + (cz[z] - lo[2])], (double)m);
+            else
+                atomicAdd(&rho[((long long)wrap(cx[x], N) * N + wrap(cy[y], N)) * N + wrap(cz[z], N)], (double)m);
+        }
+    }
+    if (!fits)
+        return;
+    __syncthreads();
+    const int ncell = sx * sy * sz;
+    for (int c = threadIdx.x; c < ncell; c += kPmBlock) {
+        const double v = tile[c];
+        if (v == 0.0)
+            continue;
+        const int x = c / (sy * sz), y = (c / sz) % sy, z = c % sz;
+        atomicAdd(&rho[((long long)wrap(lo[0] + x, N) * N + wrap(lo[1] + y, N)) * N + wrap(lo[2] + z, N)], v);
     }
 }
 
@@ -112,7 +169,31 @@ __global__ __launch_bounds__(kPmBlock) void k_pm_green(int N, double ismth2, dou
     spec[e] = v;
 }
 
-__global__ __launch_bounds__(kPmBlock) void k_pm_interp(PmArgs a, const double *__restrict__ pot,
+// grad[d][cell] = f1 invx (p[+1] - p[-1]) - f2 invx (p[+2] - p[-2]) along d:
+// the reference evaluates this difference at each of a particle's 8 CIC
+// cells (src/partmesh.c:2694-2922); it depends on the cell alone
+__global__ __launch_bounds__(kPmBlock) void k_pm_gradient(const double *__restrict__ pot, int N, double invx,
+                                                          double *__restrict__ grad)
+{
+    const long long ncell = (long long)N * N * N;
+    const long long e = (long long)blockIdx.x * kPmBlock + threadIdx.x;
+    if (e >= ncell)
+        return;
+    const int q[3] = {(int)(e / ((long long)N * N)), (int)((e / N) % N), (int)(e % N)};
+    const double f1 = (double)(float)(4.0 / 3.0), f2 = (double)(float)(1.0 / 6.0);
+    for (int d = 0; d < 3; ++d) {
+        auto at = [&](int o) {
+            int r[3] = {q[0], q[1], q[2]};
+            r[d] = wrap(r[d] + o, N);
+            return pot[((long long)r[0] * N + r[1]) * N + r[2]];
+        };
+        grad[d * ncell + e] = f1 * invx * (at(1) - at(-1)) - f2 * invx * (at(2) - at(-2));
+    }
+}
+
+// acc[d][p] = sum over the 8 CIC cells of wx wy wz grad[d][cell], in the
+// reference's corner order
+__global__ __launch_bounds__(kPmBlock) void k_pm_interp(PmArgs a, const double *__restrict__ grad,
                                                         float *__restrict__ acc)
 {
     const long long p = (long long)blockIdx.x * kPmBlock + threadIdx.x;
@@ -127,25 +208,21 @@ __global__ __launch_bounds__(kPmBlock) void k_pm_interp(PmArgs a, const double *
     Axis ax[3];
     particle_axes(a, p, ax);
     const int N = a.nside;
-    const double invx = (double)(float)(0.5 * (double)N / (double)a.box);
-    const double f1 = (double)(float)(4.0 / 3.0), f2 = (double)(float)(1.0 / 6.0);
-    const int c3[3][2] = {{ax[0].i, ax[0].ii}, {ax[1].i, ax[1].ii}, {ax[2].i, ax[2].ii}};
-    const double w3[3][2] = {{ax[0].wn, ax[0].w}, {ax[1].wn, ax[1].w}, {ax[2].wn, ax[2].w}};
-    for (int d = 0; d < 3; ++d) {
-        double s = 0.0;
-        for (int c = 0; c < 8; ++c) {
-            const int sel[3] = {c & 1, (c >> 1) & 1, c >> 2};
-            int q[3] = {c3[0][sel[0]], c3[1][sel[1]], c3[2][sel[2]]};
-            auto at = [&](int o) {
-                int r[3] = {q[0], q[1], q[2]};
-                r[d] += o;
-                return pot[((long long)wrap(r[0], N) * N + wrap(r[1], N)) * N + wrap(r[2], N)];
-            };
-            const double dp = f1 * invx * (at(1) - at(-1)) - f2 * invx * (at(2) - at(-2));
-            s += w3[0][sel[0]] * w3[1][sel[1]] * w3[2][sel[2]] * dp;
-        }
-        acc[d * a.n + p] = (float)s;
+    const long long ncell = (long long)N * N * N;
+    const int cx[2] = {wrap(ax[0].i, N), wrap(ax[0].ii, N)};
+    const int cy[2] = {wrap(ax[1].i, N), wrap(ax[1].ii, N)};
+    const int cz[2] = {wrap(ax[2].i, N), wrap(ax[2].ii, N)};
+    const double wx[2] = {ax[0].wn, ax[0].w}, wy[2] = {ax[1].wn, ax[1].w}, wz[2] = {ax[2].wn, ax[2].w};
+    double s[3] = {0.0, 0.0, 0.0};
+    for (int c = 0; c < 8; ++c) {
+        const int x = c & 1, y = (c >> 1) & 1, z = c >> 2;
+        const long long cell = ((long long)cx[x] * N + cy[y]) * N + cz[z];
+        const double w = wx[x] * wy[y] * wz[z];
+        for (int d = 0; d < 3; ++d)
+            s[d] += w * grad[d * ncell + cell];
     }
+    for (int d = 0; d < 3; ++d)
+        acc[d * a.n + p] = (float)s[d];
 }
 
 __global__ void k_pm_zero(double *p, long long n)
@@ -162,7 +239,7 @@ inline unsigned blocks_of(long long n) { return (unsigned)((n + kPmBlock - 1) / 
 hipError_t pm_deposit(const PmArgs &a, double *rho, hipStream_t st)
 {
     if (a.n > 0)
-        k_pm_deposit<<<blocks_of(a.n), kPmBlock, 0, st>>>(a, rho);
+        k_pm_deposit<<<(unsigned)((a.n + kPmChunk - 1) / kPmChunk), kPmBlock, 0, st>>>(a, rho);
     return hipGetLastError();
 }
 
@@ -178,10 +255,13 @@ hipError_t pm_green(int nside, float split_radius, float box, double2 *spec, hip
     return hipGetLastError();
 }
 
-hipError_t pm_interp(const PmArgs &a, const double *pot, float *acc, hipStream_t st)
+hipError_t pm_interp(const PmArgs &a, const double *pot, double *grad, float *acc, hipStream_t st)
 {
+    const int N = a.nside;
+    const double invx = (double)(float)(0.5 * (double)N / (double)a.box);   // src/partmesh.c:2596, FP32
+    k_pm_gradient<<<blocks_of((long long)N * N * N), kPmBlock, 0, st>>>(pot, N, invx, grad);
     if (a.n > 0)
-        k_pm_interp<<<blocks_of(a.n), kPmBlock, 0, st>>>(a, pot, acc);
+        k_pm_interp<<<blocks_of(a.n), kPmBlock, 0, st>>>(a, grad, acc);
     return hipGetLastError();
 }
 
