@@ -533,20 +533,28 @@ def main():
         # the oracle's transform of the device mass mesh, and its gradient and
         # interpolation at 20,000 sampled particles of the device potential
         pot = ctx.pm_mesh(1)
+        t1 = time.perf_counter()
         pot_ref = oracle_pm.potential(rho, np.float32(w.rs), np.float32(box))
+        cpu_pot_s = time.perf_counter() - t1
         pot_err = float(np.abs(pot - pot_ref).max() / np.abs(pot_ref).max())
         del pot_ref, rho
         pick = np.random.default_rng(5).choice(np.nonzero(loc)[0], size=min(20000, int(loc.sum())), replace=False)
+        t1 = time.perf_counter()
         a_ref = oracle_pm.accelerations(pot, m["posi"][pick], m["int2pos"], np.float32(box))
+        cpu_interp_rate = pick.size / (time.perf_counter() - t1)
         acc_err = float(np.abs(acc_pm[pick] - a_ref).max() / np.abs(a_ref).max())
         del pot
         pm = {"nside": nside, "box": box, "nlocal": int(loc.sum()), "device_ms": pm_dev, "call_ms": pm_call,
               "mass_rel_err": mass_err, "momentum_rel": momentum, "potential_rel_err_vs_oracle_fft": pot_err,
-              "acc_rel_err_vs_oracle_sample": acc_err}
+              "acc_rel_err_vs_oracle_sample": acc_err,
+              "cpu_oracle_potential_s": cpu_pot_s, "cpu_oracle_interp_particles_per_s": cpu_interp_rate,
+              "cpu_sample": "numpy oracle, one process: FFT + Green's function + inverse FFT of the whole mesh; "
+                            "gradient + interpolation at the 20,000 checked particles"}
         log(f"[bench] pm: {int(loc.sum())} particles on a {nside}^3 mesh: {pm_dev:.2f} ms device "
             f"(deposit, FFT, Green, inverse FFT, gradient + interpolation), {pm_call:.1f} ms per call with the "
             f"upload; mass {mass_err:.1e}, |sum a| / sum |a| {momentum:.1e}, potential vs oracle FFT {pot_err:.1e}, "
-            f"acc vs oracle (20000 particles) {acc_err:.1e}")
+            f"acc vs oracle (20000 particles) {acc_err:.1e}; CPU (numpy oracle): potential {cpu_pot_s:.2f} s, "
+            f"interpolation {cpu_interp_rate / 1e6:.2f} M particles/s")
     ctx.close()
 
     copy_cal = None
