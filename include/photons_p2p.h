@@ -240,7 +240,15 @@ int pp2p_get_expansions(pp2p_ctx *ctx, float *M, float *L);
  * *device_ms: deposit to interpolation, on the device.  Synchronous. */
 int pp2p_pm_forces(pp2p_ctx *ctx, long long npart, const int *posi, const int *tag, double int2pos,
                    float box, int nside, float split_radius, float mass, float *acc, double *device_ms);
-/* The last pp2p_pm_forces call's mesh: which = 0 the deposited mass per cell,
+/* The same particle mesh on the particles of the last pp2p_build_tree (its
+ * grid positions and tags stay on the device; int2pos is the build's): the
+ * local particles' accelerations are ADDED, in FP64, to the accumulator that
+ * the P2P force and the multipole pass fill, so that one accumulator holds the
+ * whole force of the split.  Asynchronous apart from the timing events;
+ * *device_ms as for pp2p_pm_forces.  Has no single counterpart in the
+ * reference, whose partmesh_kick reads part[] on the host. */
+int pp2p_pm_add_tree(pp2p_ctx *ctx, float box, int nside, float split_radius, float mass, double *device_ms);
+/* The last particle-mesh call's mesh: which = 0 the deposited mass per cell,
  * 1 the potential; nside^3 doubles, C order [x][y][z]. */
 int pp2p_pm_get_mesh(pp2p_ctx *ctx, int which, double *out);
 

@@ -270,7 +270,7 @@ __device__ int row_ranges(const RowView &v, int r, long long split, int tgt, int
             const unsigned pb = P[beg];
             int lo = beg + 1, hi = k1;
             while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
+                const int mid = lo + ((hi - lo) >> 1);      // lo + hi overflows above 2^30 tasks
                 if ((long long)(P[mid + 1] - pb) * wi > split)
                     hi = mid;
                 else

@@ -162,6 +162,7 @@ struct pp2p_ctx {
     std::vector<TreeLevel> tl;
     DevBuf<int> tx0, tx1, tx2, tidx, tsegof, tsmall, tsx, ttbl, ttag, tposi;
     DevBuf<int> tnodef, tleaff, tnoff, tloff;
+    double tree_int2pos = 0.0;  // grid step of the last pp2p_build_tree
     bool tree_ready = false;    // walk arrays, tidx and rng hold the last pp2p_build_tree
     int tree_levels = 0;        // levels of that tree (tl[0 .. tree_levels))
 
@@ -839,6 +840,7 @@ int pp2p_build_tree(pp2p_ctx *c, long long n, const int *posi, const int *tag, d
         return st;
     c->walk_nnode = nnode;
     c->tree_ready = true;
+    c->tree_int2pos = int2pos;
     c->tree_levels = last + 1;
     c->fmm_ready = false;
     c->walk_m2l_n = -1;
@@ -1667,6 +1669,53 @@ static int pm_plans(pp2p_ctx *c, int nside)
     return PP2P_OK;
 }
 
+static int pm_check(float box, int nside, float split_radius, float mass, double int2pos)
+{
+    if (nside < 8 || nside > 1024 || (nside & 1))
+        return fail(PP2P_ERR_ARG, "mesh size must be even and in [8, 1024]");
+    if (!(box > 0.f) || std::isinf(box) || !(int2pos > 0.0) || std::isinf(int2pos) || !(split_radius >= 0.f) ||
+        std::isinf(split_radius) || std::isnan(mass) || std::isinf(mass))
+        return fail(PP2P_ERR_ARG, "invalid box, grid step, split radius or mass");
+    return PP2P_OK;
+}
+
+// Deposit, transforms, gradient and interpolation of a on the library stream
+// (acc32: store FP32 SoA; else add into acc64); *device_ms: their device time.
+static int pm_run(pp2p_ctx *c, const PmArgs &a, float split_radius, float *acc32, double *acc64, double *device_ms)
+{
+    int st;
+    const hipStream_t sm = c->stream;
+    const int nside = a.nside;
+    const long long ncell = (long long)nside * nside * nside;
+    const long long nspec = (long long)nside * nside * (nside / 2 + 1);
+    c->pm_ready = false;
+    if ((st = pm_plans(c, nside)))
+        return st;
+    HIPCHK(c->pm_rho.reserve(ncell));
+    HIPCHK(c->pm_pot.reserve(ncell));
+    HIPCHK(c->pm_grad.reserve(3 * ncell));
+    HIPCHK(c->pm_spec.reserve(nspec));
+    if (!c->ev_m0) {
+        HIPCHK(hipEventCreate(&c->ev_m0));
+        HIPCHK(hipEventCreate(&c->ev_m1));
+    }
+    HIPCHK(hipEventRecord(c->ev_m0, sm));
+    HIPCHK(pm_zero(c->pm_rho.p, ncell, sm));
+    HIPCHK(pm_deposit(a, c->pm_rho.p, sm));
+    FFTCHK(hipfftExecD2Z(c->pm_fwd, c->pm_rho.p, reinterpret_cast<hipfftDoubleComplex *>(c->pm_spec.p)));
+    HIPCHK(pm_green(nside, split_radius, a.box, c->pm_spec.p, sm));
+    FFTCHK(hipfftExecZ2D(c->pm_inv, reinterpret_cast<hipfftDoubleComplex *>(c->pm_spec.p), c->pm_pot.p));
+    HIPCHK(pm_interp(a, c->pm_pot.p, c->pm_grad.p, acc32, acc64, sm));
+    HIPCHK(hipEventRecord(c->ev_m1, sm));
+    HIPCHK(hipEventSynchronize(c->ev_m1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev_m0, c->ev_m1));
+    if (device_ms)
+        *device_ms = ms;
+    c->pm_ready = true;
+    return PP2P_OK;
+}
+
 int pp2p_pm_forces(pp2p_ctx *c, long long npart, const int *posi, const int *tag, double int2pos, float box,
                    int nside, float split_radius, float mass, float *acc, double *device_ms)
 {
@@ -1675,53 +1724,43 @@ int pp2p_pm_forces(pp2p_ctx *c, long long npart, const int *posi, const int *tag
         return st;
     if (npart < 0 || (npart > 0 && (!posi || !acc)))
         return fail(PP2P_ERR_ARG, "invalid particle arrays");
-    if (nside < 8 || nside > 1024 || (nside & 1))
-        return fail(PP2P_ERR_ARG, "mesh size must be even and in [8, 1024]");
-    if (!(box > 0.f) || std::isinf(box) || !(int2pos > 0.0) || std::isinf(int2pos) || !(split_radius >= 0.f) ||
-        std::isinf(split_radius) || std::isnan(mass) || std::isinf(mass))
-        return fail(PP2P_ERR_ARG, "invalid box, grid step, split radius or mass");
+    if ((st = pm_check(box, nside, split_radius, mass, int2pos)))
+        return st;
     if (npart > (1ll << 31) - 1)
         return fail(PP2P_ERR_ARG, "more than 2^31-1 particles");
     const hipStream_t sm = c->stream;
-    const long long ncell = (long long)nside * nside * nside;
-    const long long nspec = (long long)nside * nside * (nside / 2 + 1);
-    c->pm_ready = false;
-    if ((st = pm_plans(c, nside)))
-        return st;
     HIPCHK(c->pm_posi.reserve(std::max(3 * npart, 1ll)));
     HIPCHK(c->pm_tag.reserve(std::max(npart, 1ll)));
-    HIPCHK(c->pm_rho.reserve(ncell));
-    HIPCHK(c->pm_pot.reserve(ncell));
-    HIPCHK(c->pm_grad.reserve(3 * ncell));
-    HIPCHK(c->pm_spec.reserve(nspec));
     HIPCHK(c->pm_acc.reserve(std::max(3 * npart, 1ll)));
-    if (!c->ev_m0) {
-        HIPCHK(hipEventCreate(&c->ev_m0));
-        HIPCHK(hipEventCreate(&c->ev_m1));
-    }
     if (npart > 0) {
         HIPCHK(hipMemcpyAsync(c->pm_posi.p, posi, sizeof(int) * 3 * npart, hipMemcpyHostToDevice, sm));
         if (tag)
             HIPCHK(hipMemcpyAsync(c->pm_tag.p, tag, sizeof(int) * npart, hipMemcpyHostToDevice, sm));
     }
-    PmArgs a{c->pm_posi.p, tag ? c->pm_tag.p : nullptr, npart, int2pos, box, mass, nside};
-    HIPCHK(hipEventRecord(c->ev_m0, sm));
-    HIPCHK(pm_zero(c->pm_rho.p, ncell, sm));
-    HIPCHK(pm_deposit(a, c->pm_rho.p, sm));
-    FFTCHK(hipfftExecD2Z(c->pm_fwd, c->pm_rho.p, reinterpret_cast<hipfftDoubleComplex *>(c->pm_spec.p)));
-    HIPCHK(pm_green(nside, split_radius, box, c->pm_spec.p, sm));
-    FFTCHK(hipfftExecZ2D(c->pm_inv, reinterpret_cast<hipfftDoubleComplex *>(c->pm_spec.p), c->pm_pot.p));
-    HIPCHK(pm_interp(a, c->pm_pot.p, c->pm_grad.p, c->pm_acc.p, sm));
-    HIPCHK(hipEventRecord(c->ev_m1, sm));
-    if (npart > 0)
+    const int *p = c->pm_posi.p;
+    PmArgs a{p, p + 1, p + 2, 3, tag ? c->pm_tag.p : nullptr, nullptr, npart, int2pos, box, mass, nside};
+    if ((st = pm_run(c, a, split_radius, c->pm_acc.p, nullptr, device_ms)))
+        return st;
+    if (npart > 0) {
         HIPCHK(hipMemcpyAsync(acc, c->pm_acc.p, sizeof(float) * 3 * npart, hipMemcpyDeviceToHost, sm));
-    HIPCHK(hipStreamSynchronize(sm));
-    float ms = 0.f;
-    HIPCHK(hipEventElapsedTime(&ms, c->ev_m0, c->ev_m1));
-    if (device_ms)
-        *device_ms = ms;
-    c->pm_ready = true;
+        HIPCHK(hipStreamSynchronize(sm));
+    }
     return PP2P_OK;
+}
+
+int pp2p_pm_add_tree(pp2p_ctx *c, float box, int nside, float split_radius, float mass, double *device_ms)
+{
+    int st = check_ctx(c);
+    if (st)
+        return st;
+    if (!c->tree_ready)
+        return fail(PP2P_ERR_STATE, "build_tree before pm_add_tree");
+    if ((st = pm_check(box, nside, split_radius, mass, c->tree_int2pos)))
+        return st;
+    // the tree's particles in tree order: coordinate planes, tags through the
+    // permutation; the accelerations go into the accumulator's planes
+    PmArgs a{c->tx0.p, c->tx1.p, c->tx2.p, 1, c->ttag.p, c->tidx.p, c->npart, c->tree_int2pos, box, mass, nside};
+    return pm_run(c, a, split_radius, nullptr, c->acc.p, device_ms);
 }
 
 int pp2p_pm_get_mesh(pp2p_ctx *c, int which, double *out)

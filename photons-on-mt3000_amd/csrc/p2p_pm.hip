@@ -54,8 +54,9 @@ __device__ __forceinline__ void particle_axes(const PmArgs &a, long long p, Axis
 {
     const float norm = (float)a.nside / a.box;
     const float delta = (float)(1.0 / (double)norm);
+    const int *pc[3] = {a.px, a.py, a.pz};
     for (int d = 0; d < 3; ++d) {
-        const float posf = (float)((double)a.posi[3 * p + d] * a.int2pos);
+        const float posf = (float)((double)pc[d][p * a.stride] * a.int2pos);
         ax[d] = cic_axis(posf, norm, delta);
     }
 }
@@ -68,9 +69,14 @@ __device__ __forceinline__ void particle_axes(const PmArgs &a, long long p, Axis
 constexpr int kPmChunk = 1024;
 constexpr int kPmTile = 4096;
 
+__device__ __forceinline__ bool is_local(const PmArgs &a, long long p)
+{
+    return !a.tag || a.tag[a.tag_idx ? a.tag_idx[p] : p] == 1;
+}
+
 __device__ __forceinline__ bool local_particle(const PmArgs &a, long long p)
 {
-    return p < a.n && (!a.tag || a.tag[p] == 1);
+    return p < a.n && is_local(a, p);
 }
 
 __global__ __launch_bounds__(kPmBlock) void k_pm_deposit(PmArgs a, double *__restrict__ rho)
@@ -194,15 +200,17 @@ __global__ __launch_bounds__(kPmBlock) void k_pm_gradient(const double *__restri
 // acc[d][p] = sum over the 8 CIC cells of wx wy wz grad[d][cell], in the
 // reference's corner order
 __global__ __launch_bounds__(kPmBlock) void k_pm_interp(PmArgs a, const double *__restrict__ grad,
-                                                        float *__restrict__ acc)
+                                                        float *__restrict__ acc, double *__restrict__ acc64)
 {
     const long long p = (long long)blockIdx.x * kPmBlock + threadIdx.x;
     if (p >= a.n)
         return;
-    if (a.tag && a.tag[p] != 1) {
-        acc[p] = 0.0f;
-        acc[a.n + p] = 0.0f;
-        acc[2 * a.n + p] = 0.0f;
+    if (!is_local(a, p)) {
+        if (acc) {
+            acc[p] = 0.0f;
+            acc[a.n + p] = 0.0f;
+            acc[2 * a.n + p] = 0.0f;
+        }
         return;
     }
     Axis ax[3];
@@ -221,8 +229,13 @@ __global__ __launch_bounds__(kPmBlock) void k_pm_interp(PmArgs a, const double *
         for (int d = 0; d < 3; ++d)
             s[d] += w * grad[d * ncell + cell];
     }
-    for (int d = 0; d < 3; ++d)
-        acc[d * a.n + p] = (float)s[d];
+    if (acc) {
+        for (int d = 0; d < 3; ++d)
+            acc[d * a.n + p] = (float)s[d];
+    } else {
+        for (int d = 0; d < 3; ++d)
+            acc64[d * a.n + p] += s[d];
+    }
 }
 
 __global__ void k_pm_zero(double *p, long long n)
@@ -255,13 +268,14 @@ hipError_t pm_green(int nside, float split_radius, float box, double2 *spec, hip
     return hipGetLastError();
 }
 
-hipError_t pm_interp(const PmArgs &a, const double *pot, double *grad, float *acc, hipStream_t st)
+hipError_t pm_interp(const PmArgs &a, const double *pot, double *grad, float *acc32, double *acc64,
+                     hipStream_t st)
 {
     const int N = a.nside;
     const double invx = (double)(float)(0.5 * (double)N / (double)a.box);   // src/partmesh.c:2596, FP32
     k_pm_gradient<<<blocks_of((long long)N * N * N), kPmBlock, 0, st>>>(pot, N, invx, grad);
     if (a.n > 0)
-        k_pm_interp<<<blocks_of(a.n), kPmBlock, 0, st>>>(a, grad, acc);
+        k_pm_interp<<<blocks_of(a.n), kPmBlock, 0, st>>>(a, grad, acc32, acc64);
     return hipGetLastError();
 }
 

@@ -61,7 +61,7 @@ CORE_SYMBOLS = (
     "pp2p_set_particles_i32", "pp2p_set_active", "pp2p_set_tasks", "pp2p_add_tasks",
     "pp2p_walk_tasks", "pp2p_get_tasks", "pp2p_build_tree", "pp2p_walk_tree", "pp2p_get_tree",
     "pp2p_m2l", "pp2p_l2p", "pp2p_fmm_short", "pp2p_get_expansions", "pp2p_get_m2l_tasks",
-    "pp2p_pm_forces", "pp2p_pm_get_mesh",
+    "pp2p_pm_forces", "pp2p_pm_add_tree", "pp2p_pm_get_mesh",
     "pp2p_prepare", "pp2p_compute", "pp2p_zero_acc", "pp2p_run_graph", "pp2p_sync",
     "pp2p_get_acc_f64", "pp2p_get_acc_f32_soa", "pp2p_list_stats", "pp2p_last_kernel_ms",
     "pp2p_last_prepare_ms", "pp2p_count_inside", "pp2p_time_steps", "pp2p_law_eval",
@@ -109,6 +109,7 @@ def load(path: str | None = None) -> ctypes.CDLL:
         "pp2p_get_m2l_tasks": (i32, [vp, ctypes.POINTER(i64), vp, vp]),
         "pp2p_pm_forces": (i32, [vp, i64, vp, vp, dbl, ctypes.c_float, i32, ctypes.c_float, ctypes.c_float, vp,
                                  ctypes.POINTER(dbl)]),
+        "pp2p_pm_add_tree": (i32, [vp, ctypes.c_float, i32, ctypes.c_float, ctypes.c_float, ctypes.POINTER(dbl)]),
         "pp2p_pm_get_mesh": (i32, [vp, i32, vp]),
         "pp2p_prepare": (i32, [vp]),
         "pp2p_compute": (i32, [vp, i32]),
@@ -370,6 +371,15 @@ class Context:
                                            ctypes.byref(ms)))
         self._pm_nside = int(nside)
         return acc.T.copy(), ms.value
+
+    def pm_add_tree(self, box: float, nside: int, split_radius: float, mass: float) -> float:
+        """Add the PM accelerations of the device tree's local particles to the
+        accumulator; returns device ms."""
+        ms = ctypes.c_double(0.0)
+        self._chk(self._lib.pp2p_pm_add_tree(self._h, float(box), int(nside), float(split_radius), float(mass),
+                                             ctypes.byref(ms)))
+        self._pm_nside = int(nside)
+        return ms.value
 
     def pm_mesh(self, which: int) -> np.ndarray:
         """The last pm_forces call's mass (0) or potential (1) mesh."""

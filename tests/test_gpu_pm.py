@@ -73,3 +73,42 @@ def test_pm_rejects_bad_arguments_on_the_host(gpu_ctx):
         gpu_ctx.pm_forces(posi, None, 1e-6, -1.0, 16, 1.0, 1.0)
     with pytest.raises(pp.P2PError):
         gpu_ctx.pm_forces(posi, None, 0.0, 100.0, 16, 1.0, 1.0)
+
+
+def test_pm_on_the_device_tree_adds_to_the_accumulator(gpu_ctx):
+    """pp2p_pm_add_tree on the particles of pp2p_build_tree (tree order, tags
+    through the permutation) adds, in FP64, what pp2p_pm_forces gives for the
+    same particles; ghosts get nothing; a second call adds it again."""
+    from photons_p2p import workloads as W
+    g = np.random.default_rng(31)
+    box, n1, N = 100.0, 20, 32
+    pos = g.random((n1 ** 3, 3)) * box
+    pos[: pos.shape[0] // 3] = np.mod(50.0 + g.normal(0, 3.0, (pos.shape[0] // 3, 3)), box)
+    rs = 1.25 * box / N
+    cr = 4.5 * rs
+    int2pos = W.int_grid(box, cr)
+    posi, box_i = W.to_grid(pos, box, int2pos)
+    gp, _ = W.ghost_images(posi, box_i, int(np.ceil(np.float32(cr) / int2pos)))
+    allp = np.concatenate([posi, gp])
+    tag = np.concatenate([np.ones(len(posi), np.int32), np.zeros(len(gp), np.int32)])
+    bdl, bdr = W.combine_bounds(allp, int2pos, box)
+    gpu_ctx.set_tuning()
+    gpu_ctx.set_law(pp.ForceLaw(eps=0.05 * rs, rs=rs, mass=0.37))
+    gpu_ctx.build_tree(allp, tag, int2pos, 16, bdl, bdr)
+    perm = gpu_ctx.get_tree()["perm"]
+    boxf = float(box_i * int2pos)
+    ref, _ = gpu_ctx.pm_forces(allp[perm], tag[perm], int2pos, boxf, N, rs, 0.37)
+    gpu_ctx.zero_acc()
+    ms = gpu_ctx.pm_add_tree(boxf, N, rs, 0.37)
+    a1 = gpu_ctx.acc()
+    gpu_ctx.pm_add_tree(boxf, N, rs, 0.37)
+    a2 = gpu_ctx.acc()
+    print(f"case: {allp.shape[0]} particles ({len(gp)} ghosts), {ms:.3f} ms", flush=True)
+    scale = np.abs(ref).max()
+    assert np.abs(a1 - ref).max() <= 1e-6 * scale          # FP32 rounding of ref
+    assert np.all(a1[tag[perm] != 1] == 0.0)
+    np.testing.assert_allclose(a2, 2 * a1, rtol=1e-12, atol=1e-12 * scale)
+    # the oracle on the local particles
+    loc = tag[perm] == 1
+    o, _, _ = op.pm_forces(allp[perm][loc], None, int2pos, np.float32(boxf), N, np.float32(rs), np.float32(0.37))
+    assert np.abs(a1[loc] - o).max() <= 1e-9 * np.abs(o).max()
