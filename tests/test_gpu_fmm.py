@@ -123,3 +123,25 @@ def test_device_walk_m2l_pairs_and_all_device_multipoles(gpu_ctx):
     assert np.all(np.abs(L_dev - L_host) <= 1e-6 * scale)
     mag = np.linalg.norm(a_host, axis=1).max()
     assert np.abs(a_dev - a_host).max() <= 1e-6 * mag
+
+
+def test_device_walk_without_well_separated_pairs(gpu_ctx):
+    """A tree too small for any well-separated pair: the device walk records
+    none, and the multipole pass over them equals the pass over an empty list."""
+    allp, tag, int2pos, maxleaf, bdl, bdr, cr, T = _case(6, 256, seed=5)      # 32 leaves, no M2L pair
+    rs = cr / 4.5
+    gpu_ctx.set_tuning()
+    gpu_ctx.set_law(pp.ForceLaw(eps=0.05 * rs, rs=rs, mass=0.5))
+    gpu_ctx.build_tree(allp, tag, int2pos, maxleaf, bdl, bdr)
+    gpu_ctx.walk_tree(np.float32(0.5), np.float32(cr))
+    mt, ms_ = gpu_ctx.get_m2l_tasks()
+    print(f"case: n1=6 maxleaf=256 m2l pairs device {mt.size} host {T['m2l_t'].size}", flush=True)
+    assert mt.size == T["m2l_t"].size == 0
+    gpu_ctx.zero_acc()
+    gpu_ctx.fmm_short()
+    a_dev = gpu_ctx.acc()
+    gpu_ctx.zero_acc()
+    gpu_ctx.fmm_short(T["m2l_t"], T["m2l_s"])
+    a_host = gpu_ctx.acc()
+    mag = max(np.linalg.norm(a_host, axis=1).max(), 1e-30)
+    assert np.abs(a_dev - a_host).max() <= 1e-6 * mag

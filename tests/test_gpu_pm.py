@@ -112,3 +112,19 @@ def test_pm_on_the_device_tree_adds_to_the_accumulator(gpu_ctx):
     loc = tag[perm] == 1
     o, _, _ = op.pm_forces(allp[perm][loc], None, int2pos, np.float32(boxf), N, np.float32(rs), np.float32(0.37))
     assert np.abs(a1[loc] - o).max() <= 1e-9 * np.abs(o).max()
+
+
+def test_pm_edge_positions(gpu_ctx):
+    """Particles on the first and last grid points of the box, on cell faces
+    and at cell centres (both CIC neighbour branches, the periodic wrap on
+    both sides)."""
+    box, N = 64.0, 16
+    h = BITS // N                                 # grid points per cell
+    c = np.array([0, 1, h // 2, h // 2 + 1, h - 1, h, BITS - h // 2, BITS - 1], np.int64)
+    g = np.random.default_rng(8)
+    posi = np.stack([g.choice(c, 200), g.choice(c, 200), g.choice(c, 200)], axis=1).astype(np.int32)
+    acc, _ = gpu_ctx.pm_forces(posi, None, box / BITS, box, N, 1.25 * box / N, 1.0)
+    ref, rho_ref, _ = op.pm_forces(posi, None, box / BITS, np.float32(box), N, np.float32(1.25 * box / N),
+                                   np.float32(1.0))
+    np.testing.assert_allclose(gpu_ctx.pm_mesh(0), rho_ref, rtol=1e-12, atol=1e-12 * rho_ref.max())
+    assert np.abs(acc - ref).max() <= 2e-6 * np.abs(ref).max()
