@@ -370,7 +370,7 @@ class Context:
                                            float(box), int(nside), float(split_radius), float(mass), _ptr(acc),
                                            ctypes.byref(ms)))
         self._pm_nside = int(nside)
-        return acc.T.copy(), ms.value
+        return acc.T, ms.value                    # (n, 3) view of the SoA result
 
     def pm_add_tree(self, box: float, nside: int, split_radius: float, mass: float) -> float:
         """Add the PM accelerations of the device tree's local particles to the
