@@ -233,7 +233,8 @@ int pp2p_get_expansions(pp2p_ctx *ctx, float *M, float *L);
  * (every particle local) or npart ints, local iff tag == 1 as the reference
  * tests part[n].tag.  box = BOXSIZE, nside = NSIDEMESH (even, 8..1024),
  * split_radius = splitRadius, mass = MASSPART.  Cell-centred CIC deposit,
- * FFT, Green's function exp(-k^2 rs^2) / k^2 with the CIC deconvolution,
+ * conversion to density (x (nside / box)^3), FFT, Green's function
+ * exp(-k^2 rs^2) / k^2 with the CIC deconvolution,
  * inverse FFT, 4-point gradient at the 8 CIC cells.  acc: 3 x npart FP32
  * (SoA, the reference's acc[3][NPART]); zero for non-local particles.  The
  * mesh, the transforms and the gradient are FP64 (the reference: FP32 mesh).
@@ -248,7 +249,8 @@ int pp2p_pm_forces(pp2p_ctx *ctx, long long npart, const int *posi, const int *t
  * *device_ms as for pp2p_pm_forces.  Has no single counterpart in the
  * reference, whose partmesh_kick reads part[] on the host. */
 int pp2p_pm_add_tree(pp2p_ctx *ctx, float box, int nside, float split_radius, float mass, double *device_ms);
-/* The last particle-mesh call's mesh: which = 0 the deposited mass per cell,
+/* The last particle-mesh call's mesh: which = 0 the deposited mass per cell
+ * (before the conversion to density, x (nside / box)^3, that the convolution applies),
  * 1 the potential; nside^3 doubles, C order [x][y][z]. */
 int pp2p_pm_get_mesh(pp2p_ctx *ctx, int which, double *out);
 

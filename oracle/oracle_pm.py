@@ -9,6 +9,8 @@ Follows the reference's partmesh_kick for INTXYZ particles on one domain
 * cell-centred CIC: i = (int)(posf * norm), w = (posf - (i + 0.5) delta) norm,
   neighbour i + 1 when w > 0, else i - 1 with w = -w (src/partmesh.c:1573-1612);
   deposit MASSPART * w_x * w_y * w_z in FP32 products (:1622-1672)
+* the mass per cell becomes a density: pmesh *= renormal, renormal =
+  (NSIDEMESH / BOXSIZE)^3 in FP32 (src/partmesh.c:1797-1807)
 * Green's function on the r2c spectrum (src/conv.f90:262-318):
   G = pref exp(-k2 ismth2) / k2 / (sinc_x sinc_y sinc_z)^4, G(0) = 0,
   ismth2 = (2 pi splitRadius / box)^2, pref = box^2 / (pi N^3), pi = 3.1415926
@@ -79,6 +81,12 @@ def deposit(posi: np.ndarray, tag: np.ndarray | None, int2pos: float, box: float
     return rho.reshape(N, N, N)
 
 
+def renormal(nside: int, box: float) -> float:
+    """(NSIDEMESH / BOXSIZE)^3 as the reference's FP32 products (src/partmesh.c:1797-1798)."""
+    r = np.float32(np.float32(nside) / np.float32(box))
+    return float(np.float32(np.float32(r * r) * r))
+
+
 def green(nside: int, split_radius: float, box: float) -> np.ndarray:
     """G on the r2c half spectrum [N][N][N/2 + 1] (src/conv.f90:262-318)."""
     N = nside
@@ -109,9 +117,10 @@ def green(nside: int, split_radius: float, box: float) -> np.ndarray:
 
 
 def potential(rho: np.ndarray, split_radius: float, box: float) -> np.ndarray:
-    """Unnormalised inverse transform of G * FFT(rho) (numpy's irfftn divides by N^3)."""
+    """Unnormalised inverse transform of G * FFT(renormal * rho), rho the mass
+    per cell (numpy's irfftn divides by N^3)."""
     N = rho.shape[0]
-    spec = np.fft.rfftn(rho) * green(N, split_radius, box)
+    spec = np.fft.rfftn(rho * renormal(N, box)) * green(N, split_radius, box)
     return np.fft.irfftn(spec, s=rho.shape, axes=(0, 1, 2)) * float(N) ** 3
 
 

@@ -19,7 +19,7 @@ struct PmArgs {
 // rho[N^3] += CIC mass of the local particles (FP32 weight products, FP64 sums).
 hipError_t pm_deposit(const PmArgs &a, double *rho, hipStream_t st);
 // spec[N][N][N/2+1] *= G(k) (the long-range Green's function with the CIC
-// deconvolution; spec[0] = 0).
+// deconvolution and the mass-per-cell-to-density factor (N / box)^3; spec[0] = 0).
 hipError_t pm_green(int nside, float split_radius, float box, double2 *spec, hipStream_t st);
 // grad[3][N^3] = 4-point gradient of pot[N^3] per cell, interpolated with the
 // CIC weights: acc32 != null stores acc32[3][n] (SoA, zero for particles that

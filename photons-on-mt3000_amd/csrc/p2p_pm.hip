@@ -7,7 +7,8 @@
 // * The deposit adds the FP32 products MASSPART * wx * wy * wz into an FP64
 //   mesh with atomics, first summed per chunk of particles in LDS (the
 //   reference sums in FP32 in particle order).
-// * The Green's function multiplies the FP64 r2c spectrum in place.
+// * The Green's function multiplies the FP64 r2c spectrum in place; it
+//   includes the reference's conversion of mass per cell to density.
 // * The acceleration: 4-point central differences of the potential at the 8
 //   CIC cells, weighted by the CIC weights, in FP64; stored as FP32 like the
 //   reference's acc arrays.
@@ -258,11 +259,16 @@ hipError_t pm_deposit(const PmArgs &a, double *rho, hipStream_t st)
 
 hipError_t pm_green(int nside, float split_radius, float box, double2 *spec, hipStream_t st)
 {
-    // src/conv.f90:258-262: ismth2 = (2 pi smooth / box)^2, pref = box^2 / (pi N^3)
+    // src/conv.f90:258-262: ismth2 = (2 pi smooth / box)^2, pref = box^2 / (pi N^3);
+    // the mesh holds mass per cell, which the reference turns into a density
+    // first: pmesh *= (NSIDEMESH / BOXSIZE)^3 in FP32 (src/partmesh.c:1797-1807),
+    // folded into G here (the transform is linear)
     const double smooth = (double)split_radius, bx = (double)box;
     double ismth2 = 2.0 * kPiRef * smooth / bx;
     ismth2 = ismth2 * ismth2;
-    const double pref = bx * bx / (kPiRef * (double)nside * (double)nside * (double)nside);
+    const float rn1 = (float)nside / box;
+    const float renormal = (rn1 * rn1) * rn1;
+    const double pref = bx * bx / (kPiRef * (double)nside * (double)nside * (double)nside) * (double)renormal;
     const long long total = (long long)nside * nside * (nside / 2 + 1);
     k_pm_green<<<blocks_of(total), kPmBlock, 0, st>>>(nside, ismth2, pref, spec);
     return hipGetLastError();

@@ -5,10 +5,11 @@ lives in 2DECOMP&FFT/FFTW outside the reference tree.  The oracle is pinned
 by what the mathematics fixes:
 * the deposit conserves mass, puts a particle at a cell centre into that cell
   alone and splits a particle on a cell face evenly;
-* the force between two particles at 5-12 cells equals the closed-form
-  long-range part of the split (the erf complement of the P2P kernel's erfc),
-  plus the linear term of the periodic background, times h^3 (the
-  reference's normalisation: the mesh holds mass per cell, pref = L^2 / (pi N^3));
+* the force between two particles equals the closed-form long-range part of
+  the split (the erf complement of the P2P kernel's erfc, with the same unit
+  gravitational constant), plus the linear term of the periodic background;
+  the mesh's mass per cell is made a density, x (N / L)^3, before the
+  convolution, as the reference does;
 * the total force vanishes (CIC deposit and interpolation with the same
   weights, an antisymmetric gradient, a symmetric Green's function)."""
 import math
@@ -67,7 +68,9 @@ def test_tags_select_the_local_particles():
 # matters (the long-range part is 40-70% of Newton's): the measured error
 # is <= 4%, while dropping the CIC deconvolution gives >= 10% and a split
 # radius 10% off >= 17%.  At 5.5-12 cells the force is nearly Newtonian and
-# pins the normalisation (h^3, pref) and the sign to 1.4%.
+# pins the normalisation (the density conversion, pref) and the sign to
+# 1.4%; box 100 on 64 cells (h = 1.5625) fails by h^3 without the density
+# conversion.
 @pytest.mark.parametrize("box,N,rlo,rhi,tol", [(100.0, 64, 5.5, 12.0, 0.02), (64.0, 64, 5.5, 12.0, 0.02),
                                                (100.0, 64, 2.0, 3.0, 0.06)])
 def test_pair_force_matches_the_long_range_closed_form(box, N, rlo, rhi, tol):
@@ -91,7 +94,7 @@ def test_pair_force_matches_the_long_range_closed_form(box, N, rlo, rhi, tol):
         rr = np.linalg.norm(rv)
         u = rr / (2 * rs)
         ana = (-m * rv / rr ** 3 * (math.erf(u) - 2 * u / math.sqrt(math.pi) * math.exp(-u * u))
-               + 4 * math.pi / 3 * m * rv / box ** 3) * h ** 3
+               + 4 * math.pi / 3 * m * rv / box ** 3)
         worst = max(worst, np.linalg.norm(f - ana) / np.linalg.norm(ana))
     assert worst < tol, worst
 
