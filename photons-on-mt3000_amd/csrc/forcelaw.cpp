@@ -105,8 +105,13 @@ bool law_setup(int softening, double eps, double rs, double ucut, int segs, LawS
             double c[4];
             cheb_interp([&](double t) { return law_F(s0 + t, ucut); }, w, 3, c);
             float *dst = &ls.table[((size_t)oct * segs + k) * 4];
-            for (int d = 0; d < 4; ++d)
+            for (int d = 0; d < 4; ++d) {
                 dst[d] = (float)c[d];
+                // the cubic coefficient grows as s^-4.5: below s ~ 3e-9 (eps / rc
+                // ~ 6e-5) it leaves the FP32 range, and the table law is unusable
+                if (!std::isfinite(dst[d]))
+                    ls.table_finite = false;
+            }
         }
     }
     // octave [1, 2) (e = 0) stays zero: the cutoff

@@ -20,6 +20,7 @@ struct LawSetup {
     int table_len = 0;           // entries
     double rc = 0.0;             // cutoff radius in input length units
     int segs = 0;                // segments per octave
+    bool table_finite = true;    // false: a cubic coefficient of a used segment overflows FP32
 };
 
 // g(u) in FP64 from libm

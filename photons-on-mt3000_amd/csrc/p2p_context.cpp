@@ -341,6 +341,9 @@ int pp2p_set_params(pp2p_ctx *c, const pp2p_params *p)
     LawSetup ls;
     if (!law_setup(p->softening, p->eps, p->rs, p->ucut, c->segs, &ls, &err))
         return fail(PP2P_ERR_ARG, err);
+    if (p->law == PP2P_LAW_TABLE && !ls.table_finite)
+        return fail(PP2P_ERR_ARG, "force law: eps / rc too small for the table law (its cubic coefficients "
+                                  "overflow FP32 below eps / rc ~ 6e-5); use the polynomial law");
     if (p->box > 0.0 && p->box < 2.0 * ls.rc)
         return fail(PP2P_ERR_ARG, "periodic box shorter than twice the cutoff radius");
     // frame change invalidates the converted positions
@@ -1631,6 +1634,9 @@ int pp2p_law_eval(const pp2p_params *p, int segs, long long n, const float *s, f
     const char *err = nullptr;
     if (!law_setup(p->softening, p->eps, p->rs, p->ucut, segs ? segs : 32, &ls, &err))
         return fail(PP2P_ERR_ARG, err);
+    if (!ls.table_finite)
+        return fail(PP2P_ERR_ARG, "force law: eps / rc too small for the table law (its cubic coefficients "
+                                  "overflow FP32 below eps / rc ~ 6e-5)");
     for (long long k = 0; k < n; ++k) {
         const float sc = std::fmin(std::fmax(s[k], 0.0f), 1.0f);     // the kernel's clamp to s <= 1
         F[k] = law_logtab_eval(ls, sc);

@@ -111,3 +111,19 @@ def test_autotune_table_roundtrip(tmp_path):
     AT.save_table({sig: {"best": {"ipl_max": 3, "split_pairs": 0, "waves_per_cu": 0, "law": 0}, "ms": 1.0}}, p)
     assert AT.lookup(sig, p)["ipl_max"] == 3
     assert AT.lookup("nope", p) is None
+
+
+def test_table_law_rejects_softening_whose_coefficients_overflow():
+    """Below eps / rc ~ 6e-5 the cubic coefficient of the smallest octave
+    (about 2.2 s^-4.5) leaves the FP32 range: the table law is refused instead
+    of evaluating to NaN; just above it every value is finite and matches F(s)."""
+    rs = 3.90625
+    rc = 2 * rs * 2.8125
+    with pytest.raises(pp.P2PError):
+        pp.law_eval(pp.ForceLaw(eps=4.5e-5 * rc, rs=rs, softening=pp.SOFT_PLUMMER), np.ones(1, np.float32))
+    eps = 8e-5 * rc
+    smin = (eps / rc) ** 2
+    s = np.geomspace(smin, 1.0 - 1e-6, 2000).astype(np.float32)
+    F, _ = pp.law_eval(pp.ForceLaw(eps=eps, rs=rs, softening=pp.SOFT_PLUMMER), s)
+    assert np.all(np.isfinite(F))
+    assert np.abs(F / F_exact(s.astype(np.float64), 2.8125) - 1).max() < 5e-7
