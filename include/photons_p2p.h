@@ -245,8 +245,8 @@ int pp2p_pm_forces(pp2p_ctx *ctx, long long npart, const int *posi, const int *t
  * grid positions and tags stay on the device; int2pos is the build's): the
  * local particles' accelerations are ADDED, in FP64, to the accumulator that
  * the P2P force and the multipole pass fill, so that one accumulator holds the
- * whole force of the split.  Asynchronous apart from the timing events;
- * *device_ms as for pp2p_pm_forces.  Has no single counterpart in the
+ * whole force of the split.  Synchronous (it waits for its closing timing
+ * event); *device_ms as for pp2p_pm_forces.  Has no single counterpart in the
  * reference, whose partmesh_kick reads part[] on the host. */
 int pp2p_pm_add_tree(pp2p_ctx *ctx, float box, int nside, float split_radius, float mass, double *device_ms);
 /* The last particle-mesh call's mesh: which = 0 the deposited mass per cell
