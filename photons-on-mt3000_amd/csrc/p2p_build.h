@@ -33,6 +33,10 @@ hipError_t csr_sort(const int *key_in, const int *val_in, int *key_out, int *val
                     void *temp, size_t *temp_bytes, hipStream_t st);
 hipError_t csr_rows(const int *key_sorted, int n, int *flag, int *rowidx, int *row_tgt, int *row_off,
                     void *temp, size_t *temp_bytes, hipStream_t st);
+// olist[ip .. ip + np) = the particles of each leaf in the order the force
+// kernel takes them (along the leaf box's longest axis; leaves of more than
+// 256 particles unchanged); ocnt / ooff = the leaves' np / ip (nleaf entries).
+hipError_t leaf_order(const float4 *pos, const int2 *rng, int nleaf, int *olist, int *ocnt, int *ooff, hipStream_t st);
 // nsx[k] = sum of the source sizes rng[src[j]].y over j < k, k = 0..n, modulo
 // 2^32 (differences within one CSR row are exact).  rocPRIM convention.
 hipError_t src_prefix(const int *src, const int2 *rng, long long n, unsigned *nsx, void *temp, size_t *temp_bytes,

@@ -159,6 +159,73 @@ __global__ __launch_bounds__(256) void k_active_emit(const unsigned char *__rest
     }
 }
 
+// ---------------------------------------------------------------- i-order
+
+// The order in which the force kernel takes a leaf's i-particles: along the
+// longest axis of the leaf's box, so that the lanes of a wave hold nearby
+// particles, whose table segments for one source are then close or equal
+// (fewer LDS bank conflicts on the coefficient gathers).  One wave per leaf;
+// leaves of more than kOrderMax particles keep their order.  The forces are
+// written to each particle's own index, so only the lane assignment changes.
+constexpr int kOrderMax = 256;
+
+__device__ __forceinline__ void order_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+__global__ __launch_bounds__(256) void k_leaf_order(const float4 *__restrict__ pos, const int2 *__restrict__ rng,
+                                                   int nleaf, int *__restrict__ olist, int *__restrict__ ocnt,
+                                                   int *__restrict__ ooff)
+{
+    __shared__ float key[4][kOrderMax];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int l = blockIdx.x * 4 + wv;
+    if (l >= nleaf)
+        return;                                      // the whole wave; no block barrier below
+    const int2 r = rng[l];
+    if (lane == 0) {
+        ocnt[l] = r.y;
+        ooff[l] = r.x;
+    }
+    if (r.y <= 1 || r.y > kOrderMax) {
+        for (int k = lane; k < r.y; k += 64)
+            olist[r.x + k] = r.x + k;
+        return;
+    }
+    float lo[3] = {3.0e38f, 3.0e38f, 3.0e38f}, hi[3] = {-3.0e38f, -3.0e38f, -3.0e38f};
+    for (int k = lane; k < r.y; k += 64) {
+        const float4 q = pos[r.x + k];
+        lo[0] = fminf(lo[0], q.x); hi[0] = fmaxf(hi[0], q.x);
+        lo[1] = fminf(lo[1], q.y); hi[1] = fmaxf(hi[1], q.y);
+        lo[2] = fminf(lo[2], q.z); hi[2] = fmaxf(hi[2], q.z);
+    }
+    for (int o = 32; o > 0; o >>= 1)
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = fminf(lo[d], __shfl_xor(lo[d], o, 64));
+            hi[d] = fmaxf(hi[d], __shfl_xor(hi[d], o, 64));
+        }
+    const float ex = hi[0] - lo[0], ey = hi[1] - lo[1], ez = hi[2] - lo[2];
+    const int axis = (ex >= ey && ex >= ez) ? 0 : (ey >= ez ? 1 : 2);
+    for (int k = lane; k < r.y; k += 64) {
+        const float4 q = pos[r.x + k];
+        key[wv][k] = axis == 0 ? q.x : (axis == 1 ? q.y : q.z);
+    }
+    order_lds_sync();
+    // rank by (key, index): a permutation of [0, n)
+    for (int k = lane; k < r.y; k += 64) {
+        const float kk = key[wv][k];
+        int rank = 0;
+        for (int j = 0; j < r.y; ++j) {
+            const float kj = key[wv][j];
+            rank += (kj < kk) || (kj == kk && j < k);
+        }
+        olist[r.x + rank] = r.x + k;
+    }
+}
+
 // ---------------------------------------------------------------- items
 
 struct RowView {
@@ -421,6 +488,15 @@ hipError_t csr_rows(const int *key_sorted, int n, int *flag, int *rowidx, int *r
     if (e != hipSuccess)
         return e;
     hipLaunchKernelGGL(k_row_emit, dim3(grid_of(n)), dim3(256), 0, st, key_sorted, flag, rowidx, n, row_tgt, row_off);
+    return hipGetLastError();
+}
+
+hipError_t leaf_order(const float4 *pos, const int2 *rng, int nleaf, int *olist, int *ocnt, int *ooff, hipStream_t st)
+{
+    if (nleaf <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_leaf_order, dim3((unsigned)((nleaf + 3) / 4)), dim3(256), 0, st, pos, rng, nleaf, olist, ocnt,
+                       ooff);
     return hipGetLastError();
 }
 

@@ -96,6 +96,9 @@ struct pp2p_ctx {
 
     // active subset
     bool has_act = false;
+    // i-particle order within leaves for the force kernel (no active mask set)
+    DevBuf<int> olist, ocnt, ooff;
+    bool order_ready = false;
     int act_level = 0;
     DevBuf<unsigned char> act;
     DevBuf<int> act_cnt, act_off, ilist;
@@ -279,6 +282,7 @@ void pp2p_destroy(pp2p_ctx *c)
     if (c->ev_m1) (void)hipEventDestroy(c->ev_m1);
     c->pm_posi.release(); c->pm_tag.release(); c->pm_rho.release(); c->pm_pot.release(); c->pm_grad.release(); c->pm_spec.release();
     c->pm_acc.release();
+    c->olist.release(); c->ocnt.release(); c->ooff.release();
     if (c->pm_fwd) (void)hipfftDestroy(c->pm_fwd);
     if (c->pm_inv) (void)hipfftDestroy(c->pm_inv);
     c->nsx.release();
@@ -385,6 +389,12 @@ static int finish_particles(pp2p_ctx *c, long long npart, int nleaf)
     if (abs)
         HIPCHK(launch_round_f32(c->x3.p, 3 * npart, c->stream));
     HIPCHK(launch_leaf_frame(c->x3.p, c->rng.p, nleaf, 1.0 / c->law.rc, abs, c->ref.p, c->pos.p, c->stream));
+    HIPCHK(c->olist.reserve(std::max<long long>(npart, 1)));
+    HIPCHK(c->ocnt.reserve(nleaf + 1));
+    HIPCHK(c->ooff.reserve(nleaf + 1));
+    HIPCHK(leaf_order(c->pos.p, c->rng.p, nleaf, c->olist.p, c->ocnt.p, c->ooff.p, c->stream));
+    HIPCHK(launch_fill_i32(c->ocnt.p + nleaf, 1, 0, c->stream));
+    c->order_ready = true;
     HIPCHK(launch_fill_f64(c->acc.p, 3 * npart, 0.0, c->stream));
     c->npart = npart;
     c->nleaf = nleaf;
@@ -1200,8 +1210,8 @@ static RowArgs row_args(pp2p_ctx *c)
     ra.src = c->src.p;
     ra.rng = c->rng.p;
     ra.nsx = c->nsx.p;
-    ra.act_cnt = c->has_act ? c->act_cnt.p : nullptr;
-    ra.act_off = c->has_act ? c->act_off.p : nullptr;
+    ra.act_cnt = c->has_act ? c->act_cnt.p : (c->order_ready ? c->ocnt.p : nullptr);
+    ra.act_off = c->has_act ? c->act_off.p : (c->order_ready ? c->ooff.p : nullptr);
     ra.nrows = c->nrows;
     // chunk_model > 0: at most that many i-particles per item (default 256)
     ra.stage_ops = c->tuning.chunk_model > 0 ? (float)c->tuning.chunk_model : -1.0f;
@@ -1215,7 +1225,7 @@ static ForceVariant variant(pp2p_ctx *c, bool queue)
     v.law = c->params.law == PP2P_LAW_POLY ? LAW_POLY : LAW_LOGTAB;
     v.soft = c->params.softening == PP2P_SOFT_PLUMMER ? SOFT_PLUMMER : SOFT_REF;
     v.fold64 = c->params.frame == PP2P_FRAME_LEAF;
-    v.indirect = c->has_act;
+    v.indirect = c->has_act || c->order_ready;
     v.queue = queue;
     return v;
 }
@@ -1347,7 +1357,7 @@ static ForceArgs force_args(pp2p_ctx *c, bool queue)
     a.ref = c->ref.p;
     a.rng = c->rng.p;
     a.src = c->src.p;
-    a.ilist = c->has_act ? c->ilist.p : nullptr;
+    a.ilist = c->has_act ? c->ilist.p : (c->order_ready ? c->olist.p : nullptr);
     a.acc = c->acc.p;
     a.npart = c->npart;
     a.box = c->params.box / c->law.rc;
