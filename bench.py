@@ -153,6 +153,109 @@ def copy_calibration(mib: int) -> dict:
     return {"mib": mib, "ms": ms, "bytes_read": mib << 20, "bytes_written": mib << 20}
 
 
+def dropin_protocol(w, threads: int, len_task: int) -> dict:
+    """One force step through the drop-in library in the reference's protocol,
+    from a host tree: the host walk hands LEN_TASK tasks at a time to
+    task_compute_p2p_gpu (src/fmm.c:500-515) while it runs, then
+    get_last_kntime, hostmalloc_out, devMemcpyDH_updateAcc, hostfree_out
+    (src/gravity.c:441-447).  The host program's globals come from
+    tests/support/libhostglobals.so (what src/global.c defines in the
+    reference), the particles in Body records, the leaves in Pack records.
+    Timed twice: with the walk and the device work overlapped (the drop-in's
+    worker computes each flush as it arrives), and with the walk finished
+    before the first flush (for comparison).  The positions stay in host
+    memory and go to the device in the step, as in the reference."""
+    import ctypes
+    import photons_p2p as pp
+    from photons_p2p import workloads as W
+    m = w.meta
+    host = ctypes.CDLL(os.path.join(ROOT, "tests", "support", "libhostglobals.so"), mode=ctypes.RTLD_GLOBAL)
+    lib = ctypes.CDLL(pp.DROPIN_LIB)
+    i32, vp = ctypes.c_int, ctypes.c_void_p
+    lib.hostmalloc_task.argtypes = [i32]
+    lib.hostmalloc_out.argtypes = [i32]
+    lib.devMemcpyDH_updateAcc.argtypes = [vp, i32, i32]
+    lib.get_last_kntime.argtypes = [i32]
+    lib.get_last_kntime.restype = ctypes.c_double
+    lib.p2p_hip_status.restype = i32
+    lib.p2p_hip_error.restype = ctypes.c_char_p
+    n, nl = int(w.npart), int(w.nleaf)
+    body = np.zeros(n, dtype=[("tag", np.int32), ("act", np.int32), ("posi", np.int32, 3), ("vel", np.float32, 3),
+                              ("id", np.uint64)])
+    assert body.itemsize == 40
+    body["tag"] = m["tag"]
+    body["posi"] = m["posi"]
+    leaves = np.zeros(nl, dtype=[("active", np.int32), ("npart", np.int32), ("ipart", np.int32),
+                                 ("rest", np.uint8, 196)])
+    assert leaves.itemsize == 208
+    leaves["ipart"] = w.leaf_ip
+    leaves["npart"] = w.leaf_np
+    packs = np.zeros(9, dtype=[("raw", np.uint8, 40)])
+    accs = [np.zeros(n, np.float32) for _ in range(3)]
+    g = lambda name, t: t.in_dll(host, name)  # noqa: E731
+    g("NPART", i32).value = n
+    g("first_leaf", i32).value = n
+    g("last_leaf", i32).value = n + nl
+    g("NPACK", i32).value = 0
+    g("MAXNPACK", i32).value = 9
+    g("SoftenScale", ctypes.c_float).value = w.eps
+    g("splitRadius", ctypes.c_float).value = w.rs
+    g("MASSPART", ctypes.c_float).value = w.mass
+    g("INT2POS", ctypes.c_double).value = m["int2pos"]
+    vp.in_dll(host, "part").value = body.ctypes.data
+    vp.in_dll(host, "leaf").value = leaves.ctypes.data - n * 208          # leaf[first_leaf + l]
+    vp.in_dll(host, "packlist").value = packs.ctypes.data
+    accp = (vp * 3).in_dll(host, "acc")
+    for d in range(3):
+        accp[d] = accs[d].ctypes.data
+    hl = W._host_lib()
+    P = lambda a: a.ctypes.data_as(vp)  # noqa: E731
+    posi = np.ascontiguousarray(m["posi"], np.int32)
+    tag = np.ascontiguousarray(m["tag"], np.int32)
+    bdl = np.asarray(m["bdl"], np.float32)
+    bdr = np.asarray(m["bdr"], np.float32)
+    t0 = time.perf_counter()
+    h = hl.phost_build_tree(n, P(posi), P(tag), m["int2pos"], m["maxleaf"], P(bdl), P(bdr), threads)
+    tree_s = time.perf_counter() - t0
+    ip = np.empty(nl, np.int32)
+    npl = np.empty(nl, np.int32)
+    hl.phost_get_leaves(h, P(ip), P(npl), None, None, None)
+    same_leaves = bool(np.array_equal(ip, w.leaf_ip) and np.array_equal(npl, w.leaf_np))
+    fn = ctypes.cast(lib.task_compute_p2p_gpu, vp)
+    res = {"len_task": len_task, "host_tree_s": tree_s, "leaves_identical": same_leaves, "host_threads": threads}
+    try:
+        for overlap in (1, 0, 1):
+            for a in accs:
+                a[:] = 0.0
+            g("NPACK", i32).value = 0
+            lib.hostmalloc_task(len_task)
+            tt = vp.in_dll(host, "task_t").value
+            ts = vp.in_dll(host, "task_s").value
+            t1 = time.perf_counter()
+            rc = hl.phost_walk_stream(h, np.float32(m["open_angle"]), np.float32(m["cutoff_radius"]), threads,
+                                      len_task, n, tt, ts, fn, overlap)
+            t2 = time.perf_counter()
+            kt = lib.get_last_kntime(0)
+            t3 = time.perf_counter()
+            lib.hostmalloc_out(n)
+            lib.devMemcpyDH_updateAcc(packs.ctypes.data, g("NPACK", i32).value, -1)
+            lib.hostfree_out()
+            t4 = time.perf_counter()
+            lib.hostfree_task()
+            st = lib.p2p_hip_status()
+            if rc != 0 or st != 0:
+                raise RuntimeError(f"drop-in step failed: {st} {lib.p2p_hip_error().decode()}")
+            key = "overlapped" if overlap else "walk_then_flushes"
+            res[key] = {"walk_and_flush_calls_ms": 1e3 * (t2 - t1), "wait_for_device_ms": 1e3 * (t3 - t2),
+                        "collect_ms": 1e3 * (t4 - t3), "step_ms": 1e3 * (t4 - t1 + tree_s),
+                        "force_kernels_ms": 1e3 * kt}
+    finally:
+        hl.phost_free(h)
+        lib.devfree()
+    res["acc"] = np.stack(accs, axis=1).astype(np.float64)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -193,6 +296,10 @@ def main():
                     help="also time the one-launch-per-target form the same way and report the ratio")
     ap.add_argument("--chunk-model", type=int, default=0, help="i-chunking: 0 default, -1 chunks of 64, >0 cost")
     ap.add_argument("--no-pm", action="store_true", help="skip the particle-mesh measurement")
+    ap.add_argument("--no-dropin", action="store_true",
+                    help="skip the drop-in protocol step (host walk streaming flushes into libp2p_hip.so)")
+    ap.add_argument("--no-active-sweep", action="store_true",
+                    help="skip the default active-subset sweep of the clustered workloads")
     ap.add_argument("--no-device-walk", action="store_true",
                     help="skip building the tree and the list again on the device (tree workloads)")
     args = ap.parse_args()
@@ -319,6 +426,8 @@ def main():
                      "oracle": "FP64 direct sum, same lists and force law"}
             log(f"[bench] FP32 absolute coordinates: median {m2:.2e} p99 {p2:.2e} max {x2:.2e}")
     sweep = None
+    if args.active_list is None and w.name.startswith("clustered") and args.active <= 0 and not args.no_active_sweep:
+        args.active_list = "1,0.1,0.01"          # the active fractions of the 512^3 configuration
     if args.active_list:
         from photons_p2p import workloads as W
         n_grid = (w.meta["n1"] ** 3) // 2 if w.name.startswith("clustered") else None
@@ -587,6 +696,25 @@ def main():
     ctx.close()
 
     copy_cal = None
+    dropin = None
+    if "tree" in w.meta and args.active <= 0 and not args.no_dropin:
+        # LEN_TASK of src/setparam.c:41 for this mesh on one rank
+        len_task = int(80000000 * (w.meta["mesh"] / 512.0) ** 3)
+        d = dropin_protocol(w, threads, max(len_task, 1 << 16))
+        acc_d = d.pop("acc")
+        if not args.no_acc:
+            mask = sample_rows(w, args.acc_rows, seed=20261202)
+            (md, pd, xd), nd = accuracy(w, acc_d, mask, threads, pos=pos_eval)
+            d["rel_err"] = {"median": md, "p99": pd, "max": xd, "particles": int(nd)}
+        del acc_d
+        d["step_host_lists_ms_before"] = lists["step_host_lists_ms"] if lists else None
+        dropin = d
+        ov, sq = d["overlapped"], d["walk_then_flushes"]
+        log(f"[bench] drop-in protocol step ({w.name}, LEN_TASK {d['len_task']}): host tree {d['host_tree_s']:.2f}s, "
+            f"then walk streaming flushes into task_compute_p2p_gpu + collect: overlapped {ov['step_ms']:.1f} ms "
+            f"(walk {ov['walk_and_flush_calls_ms']:.1f} ms, then waited {ov['wait_for_device_ms']:.1f} ms), "
+            f"walk first {sq['step_ms']:.1f} ms; force kernels {ov['force_kernels_ms']:.1f} ms"
+            + (f"; p99 {d['rel_err']['p99']:.2e}" if "rel_err" in d else ""))
     if args.copy_calibration:
         copy_cal = copy_calibration(args.copy_calibration)
 
@@ -628,6 +756,7 @@ def main():
         "active_sweep": sweep,
         "copy_calibration": copy_cal,
         "lists": lists,
+        "dropin_step": dropin,
         "m2l": m2l,
         "pm": pm,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",

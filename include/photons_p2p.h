@@ -113,7 +113,9 @@ int pp2p_get_tuning(pp2p_ctx *ctx, pp2p_tuning *tuning);
 
 /*
  * Particles and leaves.  Leaf l owns particles [leaf_ip[l], leaf_ip[l] +
- * leaf_np[l]) (any size; ranges of different leaves need not be disjoint).  Positions are FP64 absolute
+ * leaf_np[l]) (any size, empty included; the ranges of non-empty leaves must
+ * be disjoint, as the kd-tree leaves are: PP2P_ERR_ARG otherwise; they need
+ * not cover every particle nor come in order).  Positions are FP64 absolute
  * coordinates, npart x 3 (AoS).  Uploads, builds the per-leaf FP64
  * reference points and the leaf-relative FP32 positions on the device and
  * zeroes the acceleration accumulator.
@@ -282,6 +284,15 @@ int pp2p_get_acc_f32_soa(pp2p_ctx *ctx, float *acc);
  * only when an active mask is set). */
 int pp2p_list_stats(pp2p_ctx *ctx, long long *ntask, long long *nrows,
                     long long *nitems, long long *npairs);
+
+/* The target CSR of the current list, as the device built it (the reference's
+ * thrust sort_by_key + reduce_by_key + exclusive_scan, src/gpu_entry.cu:
+ * 132-138): key_sorted and src are task_t and task_s stably sorted by target
+ * (ntask each), row_tgt the distinct targets in increasing order (nrows),
+ * row_off the exclusive prefix of their task counts with the total last
+ * (nrows + 1).  Each pointer nullable; *nrows (nullable) receives the row
+ * count.  Builds the CSR when the list changed.  Synchronous. */
+int pp2p_get_csr(pp2p_ctx *ctx, long long *nrows, int *key_sorted, int *src, int *row_tgt, int *row_off);
 
 /* Number of listed pair-interactions with r_eff inside the cutoff (u <= ucut),
  * counted on the device by a dedicated pass over the prepared list; the

@@ -81,11 +81,11 @@ bool law_setup(int softening, double eps, double rs, double ucut, int segs, LawS
         *err = "force law: softening length at or beyond the cutoff radius";
         return false;
     }
-    if (-e_lo + 1 > kLogOctaves) {
-        *err = "force law: eps / rc below 2^-15.5, outside the table range";
+    ls.lc.smin = (float)smin;
+    if (!(ls.lc.smin > 0.0f)) {
+        *err = "force law: eps / rc underflows FP32";
         return false;
     }
-    ls.lc.smin = (float)smin;
     int lg = 0;
     while ((1 << lg) < segs)
         ++lg;
@@ -97,7 +97,12 @@ bool law_setup(int softening, double eps, double rs, double ucut, int segs, LawS
     // LOGTAB: entry index = (biased exponent mod 32) * segs + top mantissa bits
     ls.table_len = kLogOctaves * segs;
     ls.table.assign((size_t)ls.table_len * 4, 0.0f);
-    for (int e = e_lo; e <= -1; ++e) {
+    // the used octaves [e_lo, -1] and the zero octave must not wrap onto each
+    // other in the cyclic table: below eps / rc = 2^-15.5 only the polynomial
+    // law applies
+    if (-e_lo + 1 > kLogOctaves)
+        ls.table_finite = false;
+    for (int e = ls.table_finite ? e_lo : 0; e <= -1; ++e) {
         const int oct = (e + 127) & (kLogOctaves - 1);
         for (int k = 0; k < segs; ++k) {
             const double s0 = std::ldexp(1.0 + (double)k / segs, e);

@@ -20,7 +20,8 @@ struct LawSetup {
     int table_len = 0;           // entries
     double rc = 0.0;             // cutoff radius in input length units
     int segs = 0;                // segments per octave
-    bool table_finite = true;    // false: a cubic coefficient of a used segment overflows FP32
+    bool table_finite = true;    // false: a cubic coefficient of a used segment overflows FP32,
+                                 // or eps / rc < 2^-15.5 (the used octaves would wrap); POLY only
 };
 
 // g(u) in FP64 from libm

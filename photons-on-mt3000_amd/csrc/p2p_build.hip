@@ -12,6 +12,7 @@
 #include <rocprim/rocprim.hpp>
 
 #include "p2p_build.h"
+#include "p2p_cut.h"
 
 namespace pp2p {
 
@@ -327,24 +328,10 @@ __device__ int row_ranges(const RowView &v, int r, long long split, int tgt, int
             ++out;
         }
     };
-    // the cut after `beg` is the first kk > beg with (pairs of [beg, kk]) * wi
-    // > split: a binary search on the prefix of the source sizes (the
-    // predicate grows with kk), as a sequential scan over the row would find
+    // the cut after `beg`: a binary search on the prefix of the source sizes
     const unsigned *P = v.nsx;
     for (;;) {
-        int cut = k1;
-        if (split > 0) {
-            const unsigned pb = P[beg];
-            int lo = beg + 1, hi = k1;
-            while (lo < hi) {
-                const int mid = lo + ((hi - lo) >> 1);      // lo + hi overflows above 2^30 tasks
-                if ((long long)(P[mid + 1] - pb) * wi > split)
-                    hi = mid;
-                else
-                    lo = mid + 1;
-            }
-            cut = lo;
-        }
+        const int cut = split > 0 ? range_cut(PrefixArray{P}, beg, k1, wi, split) : k1;
         ++nr;
         if (cut >= k1) {
             if (EMIT)

@@ -347,7 +347,8 @@ int pp2p_set_params(pp2p_ctx *c, const pp2p_params *p)
         return fail(PP2P_ERR_ARG, err);
     if (p->law == PP2P_LAW_TABLE && !ls.table_finite)
         return fail(PP2P_ERR_ARG, "force law: eps / rc too small for the table law (its cubic coefficients "
-                                  "overflow FP32 below eps / rc ~ 6e-5); use the polynomial law");
+                                  "overflow FP32 below eps / rc ~ 6e-5, its octaves wrap below 2^-15.5); "
+                                  "use the polynomial law");
     if (p->box > 0.0 && p->box < 2.0 * ls.rc)
         return fail(PP2P_ERR_ARG, "periodic box shorter than twice the cutoff radius");
     // frame change invalidates the converted positions
@@ -373,6 +374,24 @@ static int upload_leaves(pp2p_ctx *c, long long npart, int nleaf, const int *lea
         if (leaf_np[l] < 0 || leaf_ip[l] < 0 || (long long)leaf_ip[l] + leaf_np[l] > npart)
             return fail(PP2P_ERR_ARG, "leaf " + std::to_string(l) + " outside the particle range");
         c->h_rng[l] = make_int2(leaf_ip[l], leaf_np[l]);
+    }
+    // Every leaf owns its particles: the leaf frames and the i-order lists are
+    // written per particle, so two leaves sharing a particle would overwrite
+    // each other's entries.  Non-empty ranges must be disjoint (the kd-tree
+    // leaves of the reference are).
+    {
+        std::vector<int> ord;
+        ord.reserve(nleaf);
+        for (int l = 0; l < nleaf; ++l)
+            if (leaf_np[l] > 0)
+                ord.push_back(l);
+        std::sort(ord.begin(), ord.end(), [&](int a, int b) { return leaf_ip[a] < leaf_ip[b]; });
+        for (size_t k = 1; k < ord.size(); ++k) {
+            const int a = ord[k - 1], b = ord[k];
+            if ((long long)leaf_ip[a] + leaf_np[a] > leaf_ip[b])
+                return fail(PP2P_ERR_ARG, "leaves " + std::to_string(a) + " and " + std::to_string(b) +
+                                              " share particles (leaf ranges must be disjoint)");
+        }
     }
     HIPCHK(c->rng.reserve(std::max(nleaf, 1)));
     if (nleaf)
@@ -1535,6 +1554,34 @@ int pp2p_list_stats(pp2p_ctx *c, long long *ntask, long long *nrows, long long *
     return PP2P_OK;
 }
 
+int pp2p_get_csr(pp2p_ctx *c, long long *nrows, int *key_sorted, int *src, int *row_tgt, int *row_off)
+{
+    int st = check_ctx(c);
+    if (st)
+        return st;
+    if (!c->have_particles)
+        return fail(PP2P_ERR_STATE, "set_particles before get_csr");
+    if (!c->csr_ready && (st = build_csr(c)))
+        return st;
+    const size_t n = (size_t)c->ntask, r = (size_t)c->nrows;
+    if (n && key_sorted)
+        HIPCHK(hipMemcpyAsync(key_sorted, c->key_sorted.p, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream));
+    if (n && src)
+        HIPCHK(hipMemcpyAsync(src, c->src.p, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream));
+    if (r && row_tgt)
+        HIPCHK(hipMemcpyAsync(row_tgt, c->row_tgt.p, sizeof(int) * r, hipMemcpyDeviceToHost, c->stream));
+    if (row_off) {
+        if (r)
+            HIPCHK(hipMemcpyAsync(row_off, c->row_off.p, sizeof(int) * (r + 1), hipMemcpyDeviceToHost, c->stream));
+        else
+            row_off[0] = 0;
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (nrows)
+        *nrows = (long long)r;
+    return PP2P_OK;
+}
+
 int pp2p_count_inside(pp2p_ctx *c, long long *n_inside)
 {
     int st = check_ctx(c);
@@ -1646,7 +1693,7 @@ int pp2p_law_eval(const pp2p_params *p, int segs, long long n, const float *s, f
         return fail(PP2P_ERR_ARG, err);
     if (!ls.table_finite)
         return fail(PP2P_ERR_ARG, "force law: eps / rc too small for the table law (its cubic coefficients "
-                                  "overflow FP32 below eps / rc ~ 6e-5)");
+                                  "overflow FP32 below eps / rc ~ 6e-5, its octaves wrap below 2^-15.5)");
     for (long long k = 0; k < n; ++k) {
         const float sc = std::fmin(std::fmax(s[k], 0.0f), 1.0f);     // the kernel's clamp to s <= 1
         F[k] = law_logtab_eval(ls, sc);

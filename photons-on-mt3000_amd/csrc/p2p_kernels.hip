@@ -340,11 +340,14 @@ __device__ __forceinline__ void process_item(const ForceArgs &a, const LawConst 
                 tile[cnt + __popcll(m & below)] = make_float4(w.x, w.y, w.z, 0.f);
             cnt += __popcll(m);
             // prefetch the next window now, so that its load overlaps the batches
-            // below; lanes past the end load the block's last particle
+            // below; lanes past the end load a particle of the block they never
+            // use.  Such a lane may land on a trailing empty source, whose base
+            // makes the index its first particle minus one: -1 when that empty
+            // source starts at particle 0, hence the clamp.
             const int J2 = J + kWave;
             if (w0 + kWave < total) {
                 locate(w0 + kWave, ks, bs);
-                qv = a.pos[bs + min(J2, total - 1)];
+                qv = a.pos[max(bs + min(J2, total - 1), 0)];
             }
             J = J2;
             wave_lds_sync();

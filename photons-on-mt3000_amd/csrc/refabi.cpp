@@ -3,15 +3,24 @@
 // The functions read the host program's particle/leaf arrays and force-law
 // globals exactly where the reference's entry points read them
 // (src/gpu_entry.cu:32-209, :211-386, src/gpu_active.impl.h:1-321,
-// src/gpu_func.impl.h:184-275, :588-730) and drive one pp2p context:
-// the first call of a step uploads the integer positions and leaf ranges,
-// every flush appends its tasks, the last call launches the persistent force
-// kernel over all of them.  Results stay on the device until
-// devMemcpyDH_updateAcc adds them into acc[].
+// src/gpu_func.impl.h:184-275, :588-730) and drive one pp2p context.
+// As in the reference, every flush of the host walk (src/fmm.c:500-515) is
+// computed while the walk goes on: the call copies the flush's tasks and
+// returns, and a worker thread of the library uploads the positions and
+// leaves (first call of a step), then each flush's tasks, builds its CSR and
+// work queue and launches the persistent force kernel, the launches adding
+// into one FP64 accumulator.  Flushes that queue up while the worker is busy
+// go to the device as one list.  Results stay on the device until
+// devMemcpyDH_updateAcc adds them into acc[]; every entry point that reads
+// results or device state first waits for the queued work.
+#include <condition_variable>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <deque>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/p2p_hip.h"
@@ -38,12 +47,13 @@ struct Step {
     pp2p_ctx *ctx = nullptr;
     int device = -1;
     int call_cnt = 0;              // calls of the current step (static in the reference)
-    long long tot_tasks = 0;
+    long long tot_tasks = 0;       // tasks handed over in the current step (caller's count)
     int active_level = -1;         // level of the current active step, -1 none
-    double last_kernel_s = 0.0;
+    double kernel_ms = 0.0;        // device time of the step's force launches, summed (worker)
+    bool failed = false;           // the step's set-up failed: its flushes are dropped (worker)
     std::vector<float> out;        // hostmalloc_out: 3 planes of the copied results
     long long out_n = 0;
-    std::vector<int> lip, lnp, tt, ts;
+    std::vector<int> lip, lnp;
     // p2ppack_compute_*: per pack results, by pack index
     std::vector<std::vector<double>> pack_acc;
     bool pack_mode = false;
@@ -52,9 +62,11 @@ struct Step {
 };
 
 Step g;
+std::mutex g_err;                  // the caller's thread and the worker both record errors
 
 void set_error(int st, const std::string &msg)
 {
+    std::lock_guard<std::mutex> lk(g_err);
     g.status = st;
     g.error = msg;
     std::fprintf(stderr, "[p2p_hip %d] %s\n", PROC_RANK, msg.c_str());
@@ -97,13 +109,18 @@ bool begin_step(bool with_act, int active_level)
     pp2p_params p;
     std::memset(&p, 0, sizeof(p));
     p.softening = PP2P_SOFT_REF;        // max(r, SoftenScale), src/gpu_kernel.impl.h:66-81
-    p.law = PP2P_LAW_TABLE;
     p.frame = PP2P_FRAME_LEAF;
     p.eps = SoftenScale;
     p.rs = splitRadius;
     p.ucut = kUcut;
     p.mass = MASSPART;
     p.box = 0.0;                        // periodic images arrive as ghost particles
+    // The table law refuses softening below eps / rc ~ 6e-5 (the cubic
+    // coefficients of its smallest octaves leave the FP32 range); the
+    // polynomial law has no such floor and takes over there.
+    p.law = PP2P_LAW_TABLE;
+    if (pp2p_law_eval(&p, 0, 0, nullptr, nullptr, nullptr) != PP2P_OK)
+        p.law = PP2P_LAW_POLY;
     if (!check(pp2p_set_params(g.ctx, &p), "set_params"))
         return false;
     const int nleaf = last_leaf - first_leaf;
@@ -129,51 +146,130 @@ bool begin_step(bool with_act, int active_level)
         if (!check(pp2p_set_active(g.ctx, a.data(), active_level), "set_active"))
             return false;
     }
-    if (!check(pp2p_set_tasks(g.ctx, 0, nullptr, nullptr), "set_tasks"))
-        return false;
-    g.tot_tasks = 0;
-    g.pack_mode = false;
-    return true;
+    return check(pp2p_set_tasks(g.ctx, 0, nullptr, nullptr), "set_tasks");
 }
 
-// leaf ids relative to first_leaf, without touching the caller's arrays
-bool add_tasks(int ntask, const int *tt, const int *ts, int ileaf)
+// One unit of device work, handed from the caller's thread to the worker.
+struct Job {
+    enum Kind { BEGIN, FLUSH } kind = FLUSH;
+    bool act = false;              // BEGIN: active step
+    int level = 0;                 // BEGIN: active level
+    std::vector<int> tt, ts;       // FLUSH: leaf pairs, ids relative to first_leaf
+};
+
+// Runs on the worker.  BEGIN: the step's set-up.  A batch of FLUSH jobs: one
+// list, one launch, added into the accumulator; the launch's device time is
+// added to the step's.
+void run_jobs(std::vector<Job> &batch)
 {
-    if (ntask <= 0)
-        return true;
-    if (!tt || !ts) {
-        set_error(PP2P_ERR_ARG, "null task arrays");
-        return false;
+    if (batch[0].kind == Job::BEGIN) {
+        g.kernel_ms = 0.0;
+        g.failed = !begin_step(batch[0].act, batch[0].level);
+        return;
     }
-    g.tt.resize(ntask);
-    g.ts.resize(ntask);
-    for (int k = 0; k < ntask; ++k) {
-        g.tt[k] = tt[k] - ileaf;
-        g.ts[k] = ts[k] - ileaf;
+    if (g.failed || !g.ctx)
+        return;
+    bool ok = check(pp2p_set_tasks(g.ctx, (long long)batch[0].tt.size(), batch[0].tt.data(), batch[0].ts.data()),
+                    "set_tasks");
+    for (size_t k = 1; ok && k < batch.size(); ++k)
+        ok = check(pp2p_add_tasks(g.ctx, (long long)batch[k].tt.size(), batch[k].tt.data(), batch[k].ts.data()),
+                   "add_tasks");
+    if (ok)
+        ok = check(pp2p_compute(g.ctx, PP2P_EXEC_QUEUE), "compute");
+    if (ok) {
+        const double ms = pp2p_last_kernel_ms(g.ctx);        // waits for this launch
+        if (ms > 0)
+            g.kernel_ms += ms;
     }
-    if (!check(pp2p_add_tasks(g.ctx, ntask, g.tt.data(), g.ts.data()), "add_tasks"))
-        return false;
-    g.tot_tasks += ntask;
-    return true;
+    if (!ok)
+        g.failed = true;
 }
 
-// launches asynchronously: the host continues (M2L, PM) while it runs
-bool finish_step()
-{
-    return check(pp2p_compute(g.ctx, PP2P_EXEC_QUEUE), "compute");
-}
+// The library's worker thread and its job queue.  The caller's thread posts
+// jobs and returns; drain() waits until every posted job has run, which
+// orders the worker's writes to g before the caller's reads.
+class Worker {
+public:
+    ~Worker() { stop(); }
+    void post(Job &&j)
+    {
+        std::lock_guard<std::mutex> lk(m_);
+        if (!th_.joinable()) {
+            quit_ = false;
+            th_ = std::thread([this] { loop(); });
+        }
+        q_.push_back(std::move(j));
+        cv_.notify_one();
+    }
+    void drain()
+    {
+        std::unique_lock<std::mutex> lk(m_);
+        idle_.wait(lk, [this] { return q_.empty() && !busy_; });
+    }
+    void stop()
+    {
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            if (!th_.joinable())
+                return;
+            quit_ = true;
+            cv_.notify_one();
+        }
+        th_.join();
+    }
+
+private:
+    void loop()
+    {
+        std::unique_lock<std::mutex> lk(m_);
+        for (;;) {
+            cv_.wait(lk, [this] { return quit_ || !q_.empty(); });
+            if (q_.empty())
+                return;                                  // quit with nothing left to run
+            std::vector<Job> batch;
+            batch.push_back(std::move(q_.front()));
+            q_.pop_front();
+            // flushes waiting behind a flush go to the device with it
+            while (batch[0].kind == Job::FLUSH && !q_.empty() && q_.front().kind == Job::FLUSH) {
+                batch.push_back(std::move(q_.front()));
+                q_.pop_front();
+            }
+            busy_ = true;
+            lk.unlock();
+            run_jobs(batch);
+            lk.lock();
+            busy_ = false;
+            if (q_.empty())
+                idle_.notify_all();
+        }
+    }
+    std::thread th_;
+    std::mutex m_;
+    std::condition_variable cv_, idle_;
+    std::deque<Job> q_;
+    bool busy_ = false, quit_ = false;
+};
+
+// constructed after g, so destroyed (joined) before it
+Worker worker;
 
 void *flush_call(int ntask, int *lt, int *ls, int last_time, bool act, int active_level)
 {
-    g.status = 0;
     const bool first = g.call_cnt == 0;
     ++g.call_cnt;
     P2PPack *pk = packlist;
     if (first) {
-        ++NPACK;                                   // src/gpu_entry.cu:66
-        if (!begin_step(act, active_level))
-            return nullptr;
+        worker.drain();                                  // nothing of an earlier step is left
+        g.status = 0;
+        ++NPACK;                                         // src/gpu_entry.cu:66
+        g.tot_tasks = 0;
+        g.pack_mode = false;
         g.active_level = act ? active_level : -1;
+        Job b;
+        b.kind = Job::BEGIN;
+        b.act = act;
+        b.level = active_level;
+        worker.post(std::move(b));
     }
     if (pk) {
         pk->ntask = ntask;
@@ -185,10 +281,26 @@ void *flush_call(int ntask, int *lt, int *ls, int last_time, bool act, int activ
         pk->npart = NPART;
         pk->rankid = 0;
     }
-    if (!add_tasks(ntask, lt, ls, first_leaf))
-        return nullptr;
+    if (ntask > 0) {
+        if (!lt || !ls) {
+            worker.drain();
+            set_error(PP2P_ERR_ARG, "null task arrays");
+        } else {
+            // leaf ids relative to first_leaf, copied: the caller refills its
+            // buffers as soon as the call returns, and they are not rebased
+            Job f;
+            f.kind = Job::FLUSH;
+            f.tt.resize(ntask);
+            f.ts.resize(ntask);
+            for (int k = 0; k < ntask; ++k) {
+                f.tt[k] = lt[k] - first_leaf;
+                f.ts[k] = ls[k] - first_leaf;
+            }
+            worker.post(std::move(f));
+            g.tot_tasks += ntask;
+        }
+    }
     if (last_time) {
-        finish_step();
         // results of the whole step live in this pack: keep it visible to
         // devMemcpyDH_updateAcc even when the last flush carried no task
         if (pk)
@@ -200,6 +312,7 @@ void *flush_call(int ntask, int *lt, int *ls, int last_time, bool act, int activ
 
 void *pack_call(P2PPack *packs, int npack, bool act, int active_level)
 {
+    worker.drain();
     g.status = 0;
     if (npack <= 0)
         return nullptr;
@@ -209,6 +322,8 @@ void *pack_call(P2PPack *packs, int npack, bool act, int active_level)
     }
     if (!begin_step(act, active_level))
         return nullptr;
+    g.tot_tasks = 0;
+    g.kernel_ms = 0.0;
     g.pack_acc.assign(npack, {});
     std::vector<double> all;
     for (int n = 0; n < npack; ++n) {
@@ -219,11 +334,23 @@ void *pack_call(P2PPack *packs, int npack, bool act, int active_level)
             set_error(PP2P_ERR_ARG, "pack particle range outside [0, NPART)");
             return nullptr;
         }
-        if (!check(pp2p_zero_acc(g.ctx), "zero_acc") || !check(pp2p_set_tasks(g.ctx, 0, nullptr, nullptr), "set_tasks"))
+        if (!pk.task_t || !pk.task_s) {
+            set_error(PP2P_ERR_ARG, "null task arrays");
             return nullptr;
-        g.tot_tasks = 0;
-        if (!add_tasks(pk.ntask, pk.task_t, pk.task_s, first_leaf) || !finish_step())
+        }
+        std::vector<int> tt(pk.ntask), ts(pk.ntask);
+        for (int k = 0; k < pk.ntask; ++k) {
+            tt[k] = pk.task_t[k] - first_leaf;
+            ts[k] = pk.task_s[k] - first_leaf;
+        }
+        if (!check(pp2p_zero_acc(g.ctx), "zero_acc") ||
+            !check(pp2p_set_tasks(g.ctx, pk.ntask, tt.data(), ts.data()), "set_tasks") ||
+            !check(pp2p_compute(g.ctx, PP2P_EXEC_QUEUE), "compute"))
             return nullptr;
+        const double ms = pp2p_last_kernel_ms(g.ctx);
+        if (ms > 0)
+            g.kernel_ms += ms;
+        g.tot_tasks += pk.ntask;
         all.resize((size_t)3 * NPART);
         if (!check(pp2p_get_acc_f64(g.ctx, all.data()), "get_acc"))
             return nullptr;
@@ -255,7 +382,10 @@ void *task_compute_p2p_gpu(int ntask, int *ltask_t, int *ltask_s, int last_time)
 void *task_compute_p2p_act_gpu(int ntask, int *ltask_t, int *ltask_s, int last_time, int active_level)
 {
     flush_call(ntask, ltask_t, ltask_s, last_time, true, active_level);
-    if (last_time && g.status == 0) {
+    if (last_time) {
+        worker.drain();
+        if (g.status != 0 || g.failed)
+            return nullptr;
         // src/gpu_active.impl.h:135-150: collect at once, level-filtered
         hostmalloc_out(NPART);
         devMemcpyDH_updateAcc(packlist, NPACK, active_level);
@@ -326,6 +456,7 @@ void dinfofree(void) {}
 
 void devfree(void)
 {
+    worker.drain();
     if (g.ctx)
         pp2p_destroy(g.ctx);
     g.ctx = nullptr;
@@ -352,9 +483,10 @@ void updateAcc_fromHostBuff(P2PPack pack, int active_level)
 
 void devMemcpyDH_updateAcc(P2PPack *packs, int npack, int active_level)
 {
+    worker.drain();
+    if (!g.ctx || !packs || (!g.pack_mode && g.failed))
+        return;                                     // a failed step keeps its status and adds nothing
     g.status = 0;
-    if (!g.ctx || !packs)
-        return;
     if (g.pack_mode) {
         for (int n = 0; n < npack && n < (int)g.pack_acc.size(); ++n) {
             if (packs[n].ntask == 0 || g.pack_acc[n].empty())
@@ -389,21 +521,31 @@ void devMemcpyDH_updateAcc(P2PPack *packs, int npack, int active_level)
 
 void sync_with_dev(void)
 {
+    worker.drain();
     if (g.ctx)
         check(pp2p_sync(g.ctx), "sync");
 }
 
+// seconds of device time of the step's force launches (one per flush batch)
 double get_last_kntime(int)
 {
+    worker.drain();
     if (!g.ctx)
         return 0.0;
     check(pp2p_sync(g.ctx), "sync");
-    const double ms = pp2p_last_kernel_ms(g.ctx);
-    g.last_kernel_s = ms > 0 ? ms * 1e-3 : 0.0;
-    return g.last_kernel_s;
+    return g.kernel_ms * 1e-3;
 }
 
-int p2p_hip_status(void) { return g.status; }
-const char *p2p_hip_error(void) { return g.error.c_str(); }
+int p2p_hip_status(void)
+{
+    worker.drain();
+    return g.status;
+}
+
+const char *p2p_hip_error(void)
+{
+    worker.drain();
+    return g.error.c_str();
+}
 
 }  // extern "C"

@@ -20,10 +20,13 @@
 // and the walk run their top levels in parallel (OpenMP tasks) and concatenate
 // the per-task outputs in depth-first order, which reproduces the sequential
 // numbering and order exactly.
+#include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <memory>
+#include <thread>
 #include <utility>
 #include <vector>
 
@@ -512,6 +515,72 @@ int phost_walk(phost_lists *h, float open_angle, float cutoff_radius, int nthrea
     h->off.assign(frames.size() + 1, 0);
     h->moff.assign(frames.size() + 1, 0);
     for (size_t k = 0; k < frames.size(); ++k) {
+        h->off[k + 1] = h->off[k] + (long long)h->ft[k].size();
+        h->moff[k + 1] = h->moff[k] + (long long)h->mt[k].size();
+    }
+    h->nthreads = nthreads;
+    return 0;
+}
+
+int phost_walk_stream(phost_lists *h, float open_angle, float cutoff_radius, int nthreads, int len_task,
+                      int leaf_base, int *buf_t, int *buf_s, phost_flush_fn flush, int overlap)
+{
+    if (!h || len_task < 1 || !buf_t || !buf_s || !flush)
+        return -1;
+    if (nthreads <= 0)
+        nthreads = omp_get_max_threads();
+    const Tree &t = h->tree;
+    std::vector<std::pair<int, int>> frames{{0, 0}};
+    expand(t, open_angle, cutoff_radius, frames, (size_t)nthreads * 64);
+    const long long nf = (long long)frames.size();
+    h->ft.assign(nf, {});
+    h->fs.assign(nf, {});
+    h->mt.assign(nf, {});
+    h->ms.assign(nf, {});
+    std::unique_ptr<std::atomic<int>[]> done(new std::atomic<int>[nf]);
+    for (long long k = 0; k < nf; ++k)
+        done[k].store(0, std::memory_order_relaxed);
+    // the walkers: frames in parallel, each marked done when its lists are whole
+    auto walk_all = [&]() {
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+        for (long long k = 0; k < nf; ++k) {
+            Walker w{t, open_angle, cutoff_radius, &h->ft[k], &h->fs[k], &h->mt[k], &h->ms[k]};
+            w.walk(frames[k].first, frames[k].second);
+            done[k].store(1, std::memory_order_release);
+        }
+    };
+    std::thread walkers;
+    if (overlap)
+        walkers = std::thread(walk_all);
+    else
+        walk_all();
+    // this thread: the frames' tasks in frame order into flushes of len_task
+    int fill = 0;
+    for (long long k = 0; k < nf; ++k) {
+        while (!done[k].load(std::memory_order_acquire))
+            std::this_thread::sleep_for(std::chrono::microseconds(20));
+        const std::vector<int> &ft = h->ft[k], &fs = h->fs[k];
+        size_t i = 0;
+        while (i < ft.size()) {
+            const size_t m = std::min(ft.size() - i, (size_t)(len_task - fill));
+            for (size_t j = 0; j < m; ++j) {
+                buf_t[fill + j] = ft[i + j] + leaf_base;
+                buf_s[fill + j] = fs[i + j] + leaf_base;
+            }
+            fill += (int)m;
+            i += m;
+            if (fill == len_task) {
+                flush(fill, buf_t, buf_s, 0);
+                fill = 0;
+            }
+        }
+    }
+    flush(fill, buf_t, buf_s, 1);
+    if (walkers.joinable())
+        walkers.join();
+    h->off.assign(nf + 1, 0);
+    h->moff.assign(nf + 1, 0);
+    for (long long k = 0; k < nf; ++k) {
         h->off[k + 1] = h->off[k] + (long long)h->ft[k].size();
         h->moff[k + 1] = h->moff[k] + (long long)h->mt[k].size();
     }

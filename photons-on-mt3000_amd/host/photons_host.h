@@ -23,6 +23,19 @@ phost_lists *phost_build(long long n, const int *posi, const int *tag, double in
 phost_lists *phost_build_tree(long long n, const int *posi, const int *tag, double int2pos, int maxleaf,
                               const float bdl[3], const float bdr[3], int nthreads);
 int phost_walk(phost_lists *h, float open_angle, float cutoff_radius, int nthreads);
+/* The walk of phost_walk with its list handed out while it runs, in the
+ * protocol of turn2compute_p2p (src/fmm.c:500-515): the frames' tasks, in
+ * the order of phost_get_tasks, fill buf_t / buf_s (len_task entries each,
+ * leaf ids + leaf_base), and flush(n, buf_t, buf_s, 0) is called from the
+ * calling thread each time they are full, then flush(rest, buf_t, buf_s, 1)
+ * once at the end; the buffers may be refilled as soon as flush returns.
+ * flush has the signature of task_compute_p2p_gpu (libp2p_hip.so), which can
+ * be passed directly.  overlap = 0 walks every frame before the first flush
+ * (for comparison).  The lists stay available as after phost_walk.  0 on
+ * success. */
+typedef void *(*phost_flush_fn)(int ntask, int *task_t, int *task_s, int last_time);
+int phost_walk_stream(phost_lists *h, float open_angle, float cutoff_radius, int nthreads, int len_task,
+                      int leaf_base, int *buf_t, int *buf_s, phost_flush_fn flush, int overlap);
 long long phost_count(const phost_lists *h, int what);
 /* perm[k] = original index of the particle at tree position k */
 void phost_get_perm(const phost_lists *h, int *perm);

@@ -64,7 +64,7 @@ CORE_SYMBOLS = (
     "pp2p_pm_forces", "pp2p_pm_add_tree", "pp2p_pm_get_mesh",
     "pp2p_prepare", "pp2p_compute", "pp2p_zero_acc", "pp2p_run_graph", "pp2p_sync",
     "pp2p_get_acc_f64", "pp2p_get_acc_f32_soa", "pp2p_list_stats", "pp2p_last_kernel_ms",
-    "pp2p_last_prepare_ms", "pp2p_count_inside", "pp2p_time_steps", "pp2p_law_eval",
+    "pp2p_last_prepare_ms", "pp2p_count_inside", "pp2p_time_steps", "pp2p_law_eval", "pp2p_get_csr",
 )
 
 _lib = None
@@ -122,6 +122,7 @@ def load(path: str | None = None) -> ctypes.CDLL:
         "pp2p_last_kernel_ms": (dbl, [vp]),
         "pp2p_last_prepare_ms": (dbl, [vp]),
         "pp2p_count_inside": (i32, [vp, ctypes.POINTER(i64)]),
+        "pp2p_get_csr": (i32, [vp, ctypes.POINTER(i64), vp, vp, vp, vp]),
         "pp2p_law_eval": (i32, [ctypes.POINTER(_Params), i32, i64, vp, vp, ctypes.POINTER(i32)]),
         "pp2p_time_steps": (i32, [vp, i32, i32, ctypes.POINTER(dbl), ctypes.POINTER(dbl)]),
     }
@@ -444,6 +445,18 @@ class Context:
         v = [ctypes.c_longlong(0) for _ in range(4)]
         self._chk(self._lib.pp2p_list_stats(self._h, *[ctypes.byref(x) for x in v]))
         return {"ntask": v[0].value, "nrows": v[1].value, "nitems": v[2].value, "npairs": v[3].value}
+
+    def csr(self) -> dict:
+        """The device's target CSR of the current list: key_sorted, src
+        (ntask each), row_tgt (nrows), row_off (nrows + 1)."""
+        n = ctypes.c_longlong(0)
+        self._chk(self._lib.pp2p_get_csr(self._h, ctypes.byref(n), None, None, None, None))
+        tt, ts = self.get_tasks()
+        out = {"key_sorted": np.empty(tt.size, np.int32), "src": np.empty(tt.size, np.int32),
+               "row_tgt": np.empty(n.value, np.int32), "row_off": np.empty(n.value + 1, np.int32)}
+        self._chk(self._lib.pp2p_get_csr(self._h, ctypes.byref(n), _ptr(out["key_sorted"]), _ptr(out["src"]),
+                                         _ptr(out["row_tgt"]), _ptr(out["row_off"])))
+        return out
 
     def count_inside(self) -> int:
         n = ctypes.c_longlong(0)

@@ -179,6 +179,9 @@ def _host_lib():
     lib.phost_build_tree.argtypes = [ctypes.c_longlong, vp, vp, ctypes.c_double, ctypes.c_int, vp, vp, ctypes.c_int]
     lib.phost_walk.restype = ctypes.c_int
     lib.phost_walk.argtypes = [vp, ctypes.c_float, ctypes.c_float, ctypes.c_int]
+    lib.phost_walk_stream.restype = ctypes.c_int
+    lib.phost_walk_stream.argtypes = [vp, ctypes.c_float, ctypes.c_float, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      vp, vp, vp, ctypes.c_int]
     lib.phost_count.restype = ctypes.c_longlong
     lib.phost_count.argtypes = [vp, ctypes.c_int]
     lib.phost_get_perm.argtypes = [vp, vp]

@@ -211,3 +211,123 @@ def test_packs_equal_single_list(libs):
     ref = _oracle(w)
     med, p99, mx = pyoracle.rel_err(H.acc_aos(), ref)
     assert p99 <= 1e-5 and mx <= 1e-4
+
+
+def test_pack_active_step_filters_levels(libs):
+    """p2ppack_compute_act_gpu (src/gpu_active.impl.h:155-321): two packs
+    splitting the targets by particle range, every pack computed, then the
+    results of particles whose act equals the level added into acc[] (the
+    filter of updateAcc_fromHostBuff); level 0 adds nothing."""
+    host, lib = libs
+    lib.p2ppack_compute_act_gpu.argtypes = [ctypes.POINTER(P2PPack), ctypes.c_int, ctypes.c_int]
+    lib.p2ppack_compute_act_gpu.restype = ctypes.c_void_p
+    w = W.config_plain(16, softening=0)
+    H = Host(host, w)
+    act = (np.random.default_rng(4).random(w.npart) < 0.3).astype(np.int32) * 3
+    np.ctypeslib.as_array(H.body)["act"] = act
+    half = w.npart // 2
+    tgt_first = w.leaf_ip[w.task_t]
+    bufs = []
+    for n, m in enumerate([tgt_first < half, tgt_first >= half]):
+        t = np.ascontiguousarray(w.task_t[m] + H.first_leaf, np.int32)
+        s = np.ascontiguousarray(w.task_s[m] + H.first_leaf, np.int32)
+        bufs.append((t, s))
+        lo = int(w.leaf_ip[w.task_t[m]].min())
+        hi = int((w.leaf_ip[w.task_t[m]] + w.leaf_np[w.task_t[m]]).max())
+        pk = H.packs[n]
+        pk.rankid, pk.ipart, pk.npart, pk.ntask = 0, lo, hi - lo, t.size
+        pk.ileaf, pk.nleaf = H.first_leaf, w.nleaf
+        pk.task_t = t.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+        pk.task_s = s.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+    lib.p2ppack_compute_act_gpu(H.packs, 2, 3)
+    assert lib.p2p_hip_status() == 0, lib.p2p_hip_error()
+    a = H.acc_aos()
+    sel = act == 3
+    assert np.all(a[~sel] == 0.0)
+    ref = _oracle(w, act=act.astype(np.uint8), level=3)
+    med, p99, mx = pyoracle.rel_err(a[sel], ref[sel])
+    print(f"pack active: {int(sel.sum())} active, median {med:.2e} p99 {p99:.2e} max {mx:.2e}")
+    assert p99 <= 1e-5 and mx <= 1e-4
+    before = a.copy()
+    lib.p2ppack_compute_act_gpu(H.packs, 2, 0)                 # level 0: the filter adds nothing
+    assert lib.p2p_hip_status() == 0, lib.p2p_hip_error()
+    assert np.array_equal(H.acc_aos(), before)
+
+
+def test_walk_streams_flushes_into_the_dropin(libs):
+    """The host walk hands its list to the drop-in while it runs
+    (phost_walk_stream with task_compute_p2p_gpu itself as the flush call, the
+    protocol of src/fmm.c:500-515, flushes of LEN_TASK tasks in the host
+    program's task_t / task_s).  The drop-in computes flushes on its worker
+    as they arrive; the collected step equals the FP64 oracle on the whole
+    list, and get_last_kntime sums the launches' device time."""
+    host, lib = libs
+    w = W.config_plain(24, softening=0)
+    H = Host(host, w)
+    m = w.meta
+    hl = W._host_lib()
+    P = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+    posi = np.ascontiguousarray(m["posi"], np.int32)
+    tag = np.ascontiguousarray(m["tag"], np.int32)
+    h = hl.phost_build_tree(posi.shape[0], P(posi), P(tag), m["int2pos"], m["maxleaf"],
+                            P(np.asarray(m["bdl"], np.float32)), P(np.asarray(m["bdr"], np.float32)), 0)
+    assert h
+    try:
+        ip = np.empty(w.nleaf, np.int32)
+        npl = np.empty(w.nleaf, np.int32)
+        assert hl.phost_count(h, 1) == w.nleaf
+        hl.phost_get_leaves(h, P(ip), P(npl), None, None, None)
+        assert np.array_equal(ip, w.leaf_ip) and np.array_equal(npl, w.leaf_np)
+        len_task = 3000
+        lib.hostmalloc_task(len_task)
+        tt, ts = H.tasks()
+        fn = ctypes.cast(lib.task_compute_p2p_gpu, ctypes.c_void_p)
+        print(f"case: streamed walk, {w.task_t.size} tasks in flushes of {len_task}", flush=True)
+        assert hl.phost_walk_stream(h, np.float32(m["open_angle"]), np.float32(m["cutoff_radius"]), 0, len_task,
+                                    H.first_leaf, ctypes.cast(tt, ctypes.c_void_p), ctypes.cast(ts, ctypes.c_void_p),
+                                    fn, 1) == 0
+        assert hl.phost_count(h, 3) == w.task_t.size
+    finally:
+        hl.phost_free(h)
+    lib.hostfree_task()
+    assert lib.p2p_hip_status() == 0, lib.p2p_hip_error()
+    t = lib.get_last_kntime(0)
+    assert t > 0.0
+    lib.hostmalloc_out(w.npart)
+    lib.devMemcpyDH_updateAcc(H.packs, H.npack(), -1)
+    lib.hostfree_out()
+    assert lib.p2p_hip_status() == 0, lib.p2p_hip_error()
+    ref = _oracle(w)
+    med, p99, mx = pyoracle.rel_err(H.acc_aos(), ref)
+    print(f"streamed drop-in: kernels {t * 1e3:.3f} ms, median {med:.2e} p99 {p99:.2e} max {mx:.2e}")
+    assert p99 <= 1e-5 and mx <= 1e-4
+
+
+def test_tiny_softening_falls_back_to_the_polynomial_law(libs):
+    """SoftenScale far below the table law's floor (eps / rc ~ 6e-5, where its
+    cubic coefficients leave the FP32 range): the drop-in takes the polynomial
+    law instead of failing, and the step matches the FP64 oracle with that
+    softening."""
+    host, lib = libs
+    w = W.config_plain(16, softening=0)
+    rc = 2 * w.rs * 2.8125
+    w.eps = 1e-6 * rc
+    p = pp.ForceLaw(eps=w.eps, rs=w.rs, softening=pp.SOFT_REF)
+    with pytest.raises(pp.P2PError):
+        pp.law_eval(p, np.ones(1, np.float32))                      # the table law refuses it
+    H = Host(host, w)
+    lib.hostmalloc_task(w.task_t.size)
+    tt, ts = H.tasks()
+    for j in range(w.task_t.size):
+        tt[j], ts[j] = int(w.task_t[j] + H.first_leaf), int(w.task_s[j] + H.first_leaf)
+    lib.task_compute_p2p_gpu(w.task_t.size, tt, ts, 1)
+    lib.hostfree_task()
+    assert lib.p2p_hip_status() == 0, lib.p2p_hip_error()
+    lib.hostmalloc_out(w.npart)
+    lib.devMemcpyDH_updateAcc(H.packs, H.npack(), -1)
+    lib.hostfree_out()
+    assert lib.p2p_hip_status() == 0, lib.p2p_hip_error()
+    ref = _oracle(w)
+    med, p99, mx = pyoracle.rel_err(H.acc_aos(), ref)
+    print(f"eps/rc 1e-6 (polynomial law): median {med:.2e} p99 {p99:.2e} max {mx:.2e}")
+    assert p99 <= 1e-5 and mx <= 1e-4

@@ -2,6 +2,9 @@
 same lists: empty and ragged inputs, leaves from 1 particle up to several
 work items, self pairs, repeated leaf pairs, periodic wrap, and the host-side
 rejection of bad input (no kernel runs on it)."""
+import glob
+import os
+
 import numpy as np
 import pytest
 
@@ -134,3 +137,84 @@ def test_bad_input_is_rejected_on_the_host(gpu_ctx):
     ref = _oracle(pos, ip, npl, tt, ts)
     a = _run(gpu_ctx, pos, ip, npl, tt, ts)
     _check(a, ref)
+
+
+def test_overlapping_leaves_are_rejected(gpu_ctx):
+    """Leaves own their particles: nested or partly overlapping non-empty
+    ranges are refused on the host (the frames and i-order lists are written
+    per particle); empty leaves may sit anywhere; the context stays usable."""
+    pos, ip, npl = _leaves([20, 20], seed=8)
+    gpu_ctx.set_tuning()
+    gpu_ctx.set_law(pp.ForceLaw(eps=EPS, rs=RS))
+    for bad_ip, bad_np in [([0, 10], [20, 20]),            # partial overlap
+                           ([0, 5], [30, 10]),             # nested
+                           ([7, 0], [10, 40])]:            # overlap, out of order
+        with pytest.raises(pp.P2PError, match="share particles"):
+            gpu_ctx.set_particles(pos, np.array(bad_ip, np.int32), np.array(bad_np, np.int32))
+    # empty leaves inside another leaf's range, in any order, are accepted
+    ip3 = np.array([20, 0, 5, 40], np.int32)
+    np3 = np.array([20, 20, 0, 0], np.int32)
+    tt = np.array([0, 1, 0, 1, 2, 1], np.int32)
+    ts = np.array([1, 0, 0, 1, 1, 2], np.int32)
+    ref = _oracle(pos, ip3, np3, tt, ts)
+    a = _run(gpu_ctx, pos, ip3, np3, tt, ts)
+    _check(a, ref)
+
+
+def test_trailing_empty_source_at_particle_zero(gpu_ctx):
+    """An empty source leaf starting at particle 0, listed last after a source
+    of more than 64 particles: lanes past the end of the staged sources meet
+    it while prefetching the next window of 64, and must stay in bounds
+    (index clamped at 0).  Results equal the oracle's; the empty leaf adds
+    nothing."""
+    pos, ip, npl = _leaves([100, 30], seed=9, spread=0.4)
+    ip2 = np.array([0, 0, 100], np.int32)                     # leaf 0 empty at particle 0
+    np2 = np.array([0, 100, 30], np.int32)
+    # host-side bounds check of the prefetch: lanes 64..127 of the second
+    # window locate source 1 (the empty leaf 0) once past 100 particles, whose
+    # base is ip - (prefix before it) = 0 - 100; J2 clamps to total - 1 = 99
+    base_empty = 0 - 100
+    assert max(base_empty + 99, 0) == 0
+    tt = np.array([2, 2, 1, 1], np.int32)
+    ts = np.array([1, 0, 2, 0], np.int32)
+    ref = _oracle(pos, ip2, np2, tt, ts)
+    for exec_form in (pp.EXEC_QUEUE, pp.EXEC_GRID):
+        a = _run(gpu_ctx, pos, ip2, np2, tt, ts, exec_form=exec_form)
+        _check(a, ref)
+
+
+GOLDEN = sorted(glob.glob(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "lists_*.npz")))
+
+
+@pytest.mark.parametrize("path", GOLDEN + ["uneven"], ids=[os.path.basename(p) for p in GOLDEN] + ["uneven"])
+def test_device_csr_equals_the_stable_sort(gpu_ctx, path):
+    """The target CSR the device builds (rocPRIM radix sort + row kernels) is
+    the one of the reference's thrust sort_by_key -> reduce_by_key ->
+    exclusive_scan (src/gpu_entry.cu:132-138), bit for bit: a stable sort of
+    the list by target (oracle_build_csr, a counting sort), the distinct
+    targets in order and their offsets.  Lists: the reference's own (golden
+    fixtures from src/fmm.c) and an uneven random list in random order."""
+    from photons_p2p import workloads as W
+    if path == "uneven":
+        w = W.uneven_pairs(seed=21, lattice=24, n_pool=3000, max_src=120, reach=4, total_pairs=40000)
+        g = np.random.default_rng(1)
+        order = g.permutation(w.task_t.size)
+        pos, ip, npl, tt, ts = w.pos, w.leaf_ip, w.leaf_np, w.task_t[order], w.task_s[order]
+    else:
+        f = np.load(path)
+        pos = f["posi"][f["perm"]].astype(np.float64) * float(f["int2pos"])
+        ip, npl, tt, ts = f["leaf_ip"], f["leaf_np"], f["task_t"], f["task_s"]
+    print(f"case: csr {path}: {tt.size} tasks, {ip.size} leaves", flush=True)
+    gpu_ctx.set_tuning()
+    gpu_ctx.set_law(pp.ForceLaw(eps=EPS, rs=RS))
+    gpu_ctx.set_particles(pos, ip, npl)
+    gpu_ctx.set_tasks(tt, ts)
+    c = gpu_ctx.csr()
+    tgt, off, src = pyoracle.build_csr(tt, ts, ip.size)
+    np.testing.assert_array_equal(c["key_sorted"], np.repeat(tgt, np.diff(off)))
+    np.testing.assert_array_equal(c["src"], src)
+    np.testing.assert_array_equal(c["row_tgt"], tgt)
+    np.testing.assert_array_equal(c["row_off"], off)
+    # and the same as numpy's stable argsort of the targets
+    o = np.argsort(tt, kind="stable")
+    np.testing.assert_array_equal(c["src"], ts[o])

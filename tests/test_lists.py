@@ -2,6 +2,7 @@
 the reference's own src/fmm.c (golden fixtures in tests/golden, and the live
 reference build in oracle/_ref when it exists), plus structural invariants.
 CPU only."""
+import ctypes
 import glob
 import os
 
@@ -247,3 +248,54 @@ def test_m2l_pairs_are_the_well_separated_frames():
     t, s_ = L["m2l_t"], L["m2l_s"]
     assert np.all(_accept_np(wid[t], wid[s_], cen[t] - cen[s_], theta, cr) == 1)
     assert np.all((t >= nleaf) | (s_ >= nleaf))          # never two leaves
+
+
+# the callback type of phost_walk_stream: task_compute_p2p_gpu's signature
+FLUSH_FN = ctypes.CFUNCTYPE(ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                            ctypes.POINTER(ctypes.c_int), ctypes.c_int)
+
+
+@pytest.mark.parametrize("overlap,len_task", [(1, 997), (0, 997), (1, 50000)])
+def test_streamed_walk_flushes_the_list_in_order(overlap, len_task):
+    """phost_walk_stream hands the walk's list out in flushes of len_task
+    tasks (the turn2compute_p2p protocol, src/fmm.c:500-515) while the walk
+    runs: concatenated, the flushes are the list of phost_walk, with the
+    leaf base added; every flush but the last is full and only the last has
+    last_time = 1."""
+    w = W.config_plain(16)
+    m = w.meta
+    lib = W._host_lib()
+    P = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+    posi = np.ascontiguousarray(m["posi"], np.int32)
+    tag = np.ascontiguousarray(m["tag"], np.int32)
+    h = lib.phost_build_tree(posi.shape[0], P(posi), P(tag), m["int2pos"], m["maxleaf"],
+                             P(np.asarray(m["bdl"], np.float32)), P(np.asarray(m["bdr"], np.float32)), 4)
+    assert h
+    got, flags = [], []
+    base = 1000
+
+    def flush(n, tt, ts, last):
+        got.append((np.ctypeslib.as_array(tt, (n,)).copy() if n else np.zeros(0, np.int32),
+                    np.ctypeslib.as_array(ts, (n,)).copy() if n else np.zeros(0, np.int32)))
+        flags.append(last)
+        return None
+
+    cb = FLUSH_FN(flush)
+    bt = np.empty(len_task, np.int32)
+    bs = np.empty(len_task, np.int32)
+    try:
+        assert lib.phost_walk_stream(h, np.float32(m["open_angle"]), np.float32(m["cutoff_radius"]), 4, len_task,
+                                     base, P(bt), P(bs), ctypes.cast(cb, ctypes.c_void_p), overlap) == 0
+        nt = lib.phost_count(h, 3)
+        tt = np.empty(nt, np.int32)
+        ts = np.empty(nt, np.int32)
+        lib.phost_get_tasks(h, P(tt), P(ts))
+    finally:
+        lib.phost_free(h)
+    # the tree here is the workload's (same input, same build): the same list
+    np.testing.assert_array_equal(tt, w.task_t)
+    np.testing.assert_array_equal(ts, w.task_s)
+    assert flags == [0] * (len(flags) - 1) + [1]
+    assert all(g[0].size == len_task for g in got[:-1]) and got[-1][0].size <= len_task
+    np.testing.assert_array_equal(np.concatenate([g[0] for g in got]), tt + base)
+    np.testing.assert_array_equal(np.concatenate([g[1] for g in got]), ts + base)
