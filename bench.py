@@ -267,7 +267,8 @@ def main():
                     help="plain workload: Plummer (eps = spacing / 40) or the reference's max(r, eps) rule "
                          "(eps = SoftenScale = 0.03 cell)")
     ap.add_argument("--exec", default="queue", choices=["queue", "grid", "per_target"])
-    ap.add_argument("--law", default="table", choices=["table", "poly"])
+    ap.add_argument("--law", default=None, choices=["table", "poly", "linear"],
+                    help="force law (default: the tuned table's entry for the workload, else table)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-acc", action="store_true", help="skip the accuracy sample")
     ap.add_argument("--acc-rows", type=int, default=300)
@@ -327,7 +328,8 @@ def main():
     if args.workload == "precision":
         pos_eval = w.pos + w.meta["offset"]      # absolute coordinates ~1e3 box lengths from the origin
     exec_form = {"queue": pp.EXEC_QUEUE, "grid": pp.EXEC_GRID, "per_target": pp.EXEC_PER_TARGET}[args.exec]
-    law = pp.LAW_TABLE if args.law == "table" else pp.LAW_POLY
+    law_ids = {"table": pp.LAW_TABLE, "poly": pp.LAW_POLY, "linear": pp.LAW_LINEAR}
+    law = law_ids[args.law or "table"]
 
     ctx = pp.Context(0)
     ctx.set_law(pp.ForceLaw(eps=w.eps, rs=w.rs, mass=w.mass, softening=w.softening, law=law, box=w.box))
@@ -355,7 +357,10 @@ def main():
     elif not explicit:
         cfg = AT.lookup(sig)
         if cfg is not None:
+            if args.law is not None:
+                cfg = {**cfg, "law": law}          # an explicit --law wins over the table's
             AT.apply(ctx, lawp, cfg)
+            law = cfg.get("law", law)
             tuned = {"source": "photons_p2p/tuned.json", "signature": sig, "best": cfg}
             log(f"[bench] tuned.json {sig}: {cfg}")
     n_list = st["npairs"]
@@ -743,7 +748,8 @@ def main():
         "tuning": ctx_tuning,
         "autotune": tuned,
         "config": {"workload": w.name, "npart": int(w.npart), "nleaf": int(w.nleaf), "ntask": int(w.task_t.size),
-                   "n_list": int(n_list), "n_in": int(n_in), "exec": args.exec, "law": args.law,
+                   "n_list": int(n_list), "n_in": int(n_in), "exec": args.exec,
+                   "law": {v: k for k, v in law_ids.items()}[law],
                    "softening": "plummer" if w.softening == 1 else "ref", "eps": w.eps, "rs": w.rs},
         "pct_fp32_peak": 100 * achieved / PEAK_FP32_TFLOPS,
         "rel_err": rel,

@@ -47,7 +47,10 @@ enum {
 enum {
     PP2P_LAW_TABLE = 0,     /* F(s) = g/r^3 as a cubic on 16 or 32 segments per binary
                                octave of s = r^2/rc^2, LDS table, no rsqrt (default) */
-    PP2P_LAW_POLY = 1       /* rsqrt, exp2 and a degree-12 polynomial, no table      */
+    PP2P_LAW_POLY = 1,      /* rsqrt, exp2 and a degree-12 polynomial, no table      */
+    PP2P_LAW_LINEAR = 2     /* F(s) = a + b s on 512 segments per binary octave of s,
+                               8-byte LDS entries, one workgroup of 16 waves per CU;
+                               needs eps / rc >= 2^-12.5 */
 };
 
 /* frame: how positions reach the FP32 pair arithmetic */
@@ -306,7 +309,8 @@ int pp2p_count_inside(pp2p_ctx *ctx, long long *n_inside);
 int pp2p_time_steps(pp2p_ctx *ctx, int exec, int nsteps, double *total_ms, double *force_ms);
 
 /* Host evaluation of the device force law, operation for operation as the
- * table law's pair function computes it in FP32: for n values of
+ * pair function of params->law computes it in FP32 (PP2P_LAW_TABLE with
+ * segs_per_octave segments, or PP2P_LAW_LINEAR): for n values of
  * s = r_eff^2 / rc^2 (already softened, 0 < s), F[k] = g(u) / s^{3/2} in
  * kernel units (rc = 1), or 0 at and beyond the cutoff.  Also returns the
  * table size in entries.  For host-side tests of the tables; no device use. */

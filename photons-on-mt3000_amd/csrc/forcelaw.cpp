@@ -93,6 +93,7 @@ bool law_setup(int softening, double eps, double rs, double ucut, int segs, LawS
     ls.lc.seg_bits = 5 + lg;
     ls.lc.tab_shift = ls.lc.seg_shift - 4;
     ls.lc.k2 = (float)(-ucut * ucut / std::log(2.0));
+    ls.lc.abs_s2 = (float)(1.0 / (ls.rc * ls.rc));
 
     // LOGTAB: entry index = (biased exponent mod 32) * segs + top mantissa bits
     ls.table_len = kLogOctaves * segs;
@@ -121,6 +122,44 @@ bool law_setup(int softening, double eps, double rs, double ucut, int segs, LawS
     }
     // octave [1, 2) (e = 0) stays zero: the cutoff
 
+    // LINEAR: a + b s on each of 512 segments per octave, the line through
+    // F at the segment's two Chebyshev points (uncentred in s: a ~ 2.5 F and
+    // b s ~ -1.5 F near the power law, so FP32 loses about 2e-7 of F to the
+    // cancellation).  Octaves by the biased exponent of the FP32 smin (the
+    // kernel's s never lies below it) up to the single entry of s = 1, where
+    // every pair at or beyond the cutoff lands; it holds zero.
+    {
+        unsigned sb;
+        std::memcpy(&sb, &ls.lc.smin, 4);
+        const int eb_lo = (int)(sb >> 23);                       // biased exponent of smin
+        const int nseg = 1 << (23 - kLinSegShift);
+        const long long entries = (long long)(127 - eb_lo) * nseg + 1;
+        ls.lin_ok = eb_lo >= 96 && (eb_lo & 31) * nseg * 8 >= kLinTableLo;
+        if (ls.lin_ok) {
+            ls.lin.assign((size_t)entries * 2, 0.0f);
+            for (int eb = eb_lo; eb < 127; ++eb) {
+                for (int k = 0; k < nseg; ++k) {
+                    const double s0 = std::ldexp(1.0 + (double)k / nseg, eb - 127);
+                    const double w = std::ldexp(1.0 / nseg, eb - 127);
+                    const double x0 = s0 + 0.5 * w * (1.0 - std::sqrt(0.5));
+                    const double x1 = s0 + 0.5 * w * (1.0 + std::sqrt(0.5));
+                    const double f0 = law_F(x0, ucut), f1 = law_F(x1, ucut);
+                    const double b = (f1 - f0) / (x1 - x0);
+                    const double a = f0 - b * x0;
+                    float *dst = &ls.lin[((size_t)(eb - eb_lo) * nseg + k) * 2];
+                    dst[0] = (float)a;
+                    dst[1] = (float)b;
+                    if (!std::isfinite(dst[0]) || !std::isfinite(dst[1]))
+                        ls.lin_ok = false;
+                }
+            }
+            // entry index of s: (biased exponent mod 32) * 512 + top 9 mantissa bits
+            ls.lc.lin_lo = (eb_lo & 31) * nseg * 8;
+        }
+        if (!ls.lin_ok)
+            ls.lin.clear();
+    }
+
     // POLY: h(r) = erfcx(ucut r) + (2/sqrt(pi)) ucut r on r in [0, 1]
     double hc[kPolyDeg + 1];
     cheb_interp(
@@ -145,6 +184,19 @@ float law_logtab_eval(const LawSetup &ls, float s)
     const float t = s - s0;
     const float *c = &ls.table[(size_t)seg * 4];
     return std::fma(std::fma(std::fma(c[3], t, c[2]), t, c[1]), t, c[0]);
+}
+
+}  // namespace pp2p
+
+namespace pp2p {
+
+float law_linear_eval(const LawSetup &ls, float s)
+{
+    unsigned bits;
+    std::memcpy(&bits, &s, 4);
+    const unsigned byte = (bits >> (kLinSegShift - 3)) & 0x1fff8u;      // the kernel's LDS address
+    const float *c = &ls.lin[(byte - (unsigned)ls.lc.lin_lo) / 4];
+    return std::fma(c[1], s, c[0]);
 }
 
 }  // namespace pp2p

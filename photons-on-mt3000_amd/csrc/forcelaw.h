@@ -22,6 +22,8 @@ struct LawSetup {
     int segs = 0;                // segments per octave
     bool table_finite = true;    // false: a cubic coefficient of a used segment overflows FP32,
                                  // or eps / rc < 2^-15.5 (the used octaves would wrap); POLY only
+    std::vector<float> lin;      // LINEAR: (a, b) per segment, from the lowest used octave to s = 1
+    bool lin_ok = false;         // false: the used octaves outgrow the LDS (eps / rc < 2^-12.5)
 };
 
 // g(u) in FP64 from libm
@@ -38,5 +40,8 @@ bool law_setup(int softening, double eps, double rs, double ucut, int segs_per_o
 // FP32 evaluation of the LOGTAB law exactly as the kernel does it (for host
 // tests): returns F for a given s already clamped by the softening rule.
 float law_logtab_eval(const LawSetup &ls, float s);
+
+// The same for the LINEAR law: F = fma(b, s, a) with the segment's (a, b).
+float law_linear_eval(const LawSetup &ls, float s);
 
 }  // namespace pp2p

@@ -51,6 +51,10 @@ __global__ __launch_bounds__(256) void k_leaf_frame(const double *__restrict__ x
     }
     if (lane == 0)
         ref[leaf] = make_double4(cx * inv_rc, cy * inv_rc, cz * inv_rc, 0.0);
+    // the absolute frame keeps the input units: the positions are the
+    // reference's float coordinates, unscaled (the kernel scales r^2)
+    if (frame_abs)
+        inv_rc = 1.0;
     for (int k = lane; k < r.y; k += 64) {
         const long long i = (long long)r.x + k;
         const double *p = x3 + 3 * i;

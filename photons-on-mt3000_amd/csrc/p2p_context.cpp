@@ -80,7 +80,8 @@ struct pp2p_ctx {
     pp2p_tuning tuning{};
     int segs = 32;
     LawSetup law;
-    DevBuf<float4> table;
+    DevBuf<float4> table;            // force-law table of the current law
+    int table_f4 = 0;                // its length in float4 units
 
     // particles
     long long npart = 0;
@@ -335,7 +336,7 @@ int pp2p_set_params(pp2p_ctx *c, const pp2p_params *p)
         return st;
     if (!p)
         return fail(PP2P_ERR_ARG, "null params");
-    if (p->law != PP2P_LAW_TABLE && p->law != PP2P_LAW_POLY)
+    if (p->law != PP2P_LAW_TABLE && p->law != PP2P_LAW_POLY && p->law != PP2P_LAW_LINEAR)
         return fail(PP2P_ERR_ARG, "unknown law");
     if (p->frame != PP2P_FRAME_LEAF && p->frame != PP2P_FRAME_ABS32)
         return fail(PP2P_ERR_ARG, "unknown frame");
@@ -349,6 +350,9 @@ int pp2p_set_params(pp2p_ctx *c, const pp2p_params *p)
         return fail(PP2P_ERR_ARG, "force law: eps / rc too small for the table law (its cubic coefficients "
                                   "overflow FP32 below eps / rc ~ 6e-5, its octaves wrap below 2^-15.5); "
                                   "use the polynomial law");
+    if (p->law == PP2P_LAW_LINEAR && !ls.lin_ok)
+        return fail(PP2P_ERR_ARG, "force law: eps / rc too small for the linear law (its octaves of s "
+                                  "outgrow the LDS below eps / rc ~ 2^-12.5); use the table or polynomial law");
     if (p->box > 0.0 && p->box < 2.0 * ls.rc)
         return fail(PP2P_ERR_ARG, "periodic box shorter than twice the cutoff radius");
     // frame change invalidates the converted positions
@@ -356,9 +360,14 @@ int pp2p_set_params(pp2p_ctx *c, const pp2p_params *p)
     c->law = ls;
     c->params = *p;
     c->have_params = true;
-    HIPCHK(c->table.reserve(ls.table_len));
-    HIPCHK(hipMemcpyAsync(c->table.p, ls.table.data(), sizeof(float) * ls.table.size(), hipMemcpyHostToDevice,
-                          c->stream));
+    // the device copy holds the table of the chosen law, in float4 units
+    std::vector<float> tab = p->law == PP2P_LAW_LINEAR ? ls.lin : ls.table;
+    c->table_f4 = (int)((tab.size() + 3) / 4);
+    tab.resize(4 * (size_t)c->table_f4, 0.0f);
+    HIPCHK(c->table.reserve(std::max(c->table_f4, 1)));
+    if (c->table_f4)
+        HIPCHK(hipMemcpyAsync(c->table.p, tab.data(), sizeof(float) * tab.size(), hipMemcpyHostToDevice,
+                              c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     if (frame_changed)
         c->have_particles = false;
@@ -1185,53 +1194,155 @@ int pp2p_get_tasks(pp2p_ctx *c, long long *ntask, int *task_t, int *task_s)
     return PP2P_OK;
 }
 
-static int build_csr(pp2p_ctx *c)
-{
-    const int n = (int)c->ntask;
-    HIPCHK(c->key_sorted.reserve(std::max(n, 1)));
-    HIPCHK(c->src.reserve(std::max(n, 1)));
-    HIPCHK(c->flag.reserve(std::max(n, 1)));
-    HIPCHK(c->rowidx.reserve(std::max(n, 1)));
-    HIPCHK(c->row_tgt.reserve(std::max(n, 1)));
-    HIPCHK(c->row_off.reserve(n + 1));
-    c->nrows = 0;
-    if (n == 0) {
-        c->csr_ready = true;
+// Device storage of one list's CSR and work items: the context's grow-only
+// buffers (pp2p_compute, pp2p_prepare), or consecutive slices of one device
+// region reserved beforehand, so that building a list allocates nothing.
+extern "C++" {
+class Space {
+public:
+    explicit Space(pp2p_ctx *c) : c_(c) {}
+    Space(pp2p_ctx *c, char *base, size_t cap) : c_(c), base_(base), cap_(cap) {}
+    bool scratch() const { return base_ != nullptr; }
+
+    // n (at least 1) elements of T: the context buffer b, grown if needed, or
+    // the next slice of the scratch
+    template <typename T>
+    int get(DevBuf<T> &b, size_t n, T **out)
+    {
+        n = std::max<size_t>(n, 1);
+        if (!scratch()) {
+            HIPCHK(b.reserve(n));
+            *out = b.p;
+            return PP2P_OK;
+        }
+        return carve(n * sizeof(T), reinterpret_cast<void **>(out));
+    }
+    // storage for the next rocPRIM call only (later slices may reuse it:
+    // everything runs in order on the context stream)
+    int temp(size_t bytes, void **out, size_t *len)
+    {
+        bytes = std::max<size_t>(bytes, 256);
+        if (!scratch()) {
+            HIPCHK(c_->temp.reserve(bytes));
+            *out = c_->temp.p;
+            *len = c_->temp.cap;
+            return PP2P_OK;
+        }
+        const size_t at = align(used_);
+        if (at + bytes > cap_)
+            return overflow(at + bytes);
+        *out = base_ + at;
+        *len = bytes;
         return PP2P_OK;
     }
+    int carve(size_t bytes, void **out)
+    {
+        const size_t at = align(used_);
+        if (at + bytes > cap_)
+            return overflow(at + bytes);
+        *out = base_ + at;
+        used_ = at + bytes;
+        return PP2P_OK;
+    }
+
+private:
+    static size_t align(size_t x) { return (x + 255) & ~(size_t)255; }
+    int overflow(size_t need) const
+    {
+        return fail(PP2P_ERR_ARG, "the list needs at least " + std::to_string(need) + " bytes of scratch, more than the " +
+                                      std::to_string(cap_) + " of the preallocated region; use pp2p_compute");
+    }
+    pp2p_ctx *c_;
+    char *base_ = nullptr;
+    size_t cap_ = 0, used_ = 0;
+};
+}  // extern "C++"
+
+// The CSR and work items of one list, wherever they live.
+struct ListView {
+    int *key_sorted = nullptr, *src = nullptr, *row_tgt = nullptr, *row_off = nullptr;
+    unsigned *nsx = nullptr;
+    Item *items_rows = nullptr, *items_q = nullptr;
+    long long ntask = 0;
+    int nrows = 0, nitems = 0;
+    long long split = 0;
+    unsigned long long pairs = 0;
+};
+
+static ListView ctx_view(pp2p_ctx *c)
+{
+    ListView L;
+    L.key_sorted = c->key_sorted.p;
+    L.src = c->src.p;
+    L.row_tgt = c->row_tgt.p;
+    L.row_off = c->row_off.p;
+    L.nsx = c->nsx.p;
+    L.items_rows = c->items_rows.p;
+    L.items_q = c->items_q.p;
+    L.ntask = c->ntask;
+    L.nrows = c->nrows;
+    L.nitems = c->nitems;
+    L.split = c->split;
+    L.pairs = c->list_pairs;
+    return L;
+}
+
+// Target CSR of the list (task_t, task_s): stable sort by target, then the
+// rows (src/gpu_entry.cu:132-138).  One host synchronisation (the row count).
+static int csr_build(pp2p_ctx *c, Space &sp, const int *tt, const int *ts, int n, ListView &L)
+{
+    int *flag = nullptr, *rowidx = nullptr;
+    int st;
+    if ((st = sp.get(c->key_sorted, n, &L.key_sorted)) || (st = sp.get(c->src, n, &L.src)) ||
+        (st = sp.get(c->flag, n, &flag)) || (st = sp.get(c->rowidx, n, &rowidx)) ||
+        (st = sp.get(c->row_tgt, n, &L.row_tgt)) || (st = sp.get(c->row_off, (size_t)n + 1, &L.row_off)))
+        return st;
+    L.ntask = n;
+    L.nrows = 0;
+    if (n == 0)
+        return PP2P_OK;
     int bits = 1;
     while ((1ll << bits) < (long long)c->nleaf)
         ++bits;
     size_t tb1 = 0, tb2 = 0;
-    HIPCHK(csr_sort(c->task_t.p, c->task_s.p, c->key_sorted.p, c->src.p, n, bits, nullptr, &tb1, c->stream));
-    HIPCHK(csr_rows(c->key_sorted.p, n, c->flag.p, c->rowidx.p, c->row_tgt.p, c->row_off.p, nullptr, &tb2,
-                    c->stream));
-    int st = ensure_temp(c, std::max(tb1, tb2));
+    HIPCHK(csr_sort(tt, ts, L.key_sorted, L.src, n, bits, nullptr, &tb1, c->stream));
+    HIPCHK(csr_rows(L.key_sorted, n, flag, rowidx, L.row_tgt, L.row_off, nullptr, &tb2, c->stream));
+    void *tmp = nullptr;
+    if ((st = sp.temp(std::max(tb1, tb2), &tmp, &tb1)))
+        return st;
+    tb2 = tb1;
+    HIPCHK(csr_sort(tt, ts, L.key_sorted, L.src, n, bits, tmp, &tb1, c->stream));
+    HIPCHK(csr_rows(L.key_sorted, n, flag, rowidx, L.row_tgt, L.row_off, tmp, &tb2, c->stream));
+    int rows = 0;
+    HIPCHK(hipMemcpyAsync(&rows, rowidx + n - 1, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    L.nrows = rows;
+    return PP2P_OK;
+}
+
+static int build_csr(pp2p_ctx *c)
+{
+    Space sp(c);
+    ListView L;
+    int st = csr_build(c, sp, c->task_t.p, c->task_s.p, (int)c->ntask, L);
     if (st)
         return st;
-    tb1 = tb2 = c->temp.cap;
-    HIPCHK(csr_sort(c->task_t.p, c->task_s.p, c->key_sorted.p, c->src.p, n, bits, c->temp.p, &tb1, c->stream));
-    HIPCHK(csr_rows(c->key_sorted.p, n, c->flag.p, c->rowidx.p, c->row_tgt.p, c->row_off.p, c->temp.p, &tb2,
-                    c->stream));
-    int rows = 0;
-    HIPCHK(hipMemcpyAsync(&rows, c->rowidx.p + n - 1, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    c->nrows = rows;
+    c->nrows = L.nrows;
     c->csr_ready = true;
     return PP2P_OK;
 }
 
-static RowArgs row_args(pp2p_ctx *c)
+static RowArgs row_args(pp2p_ctx *c, const ListView &L)
 {
     RowArgs ra;
-    ra.row_tgt = c->row_tgt.p;
-    ra.row_off = c->row_off.p;
-    ra.src = c->src.p;
+    ra.row_tgt = L.row_tgt;
+    ra.row_off = L.row_off;
+    ra.src = L.src;
     ra.rng = c->rng.p;
-    ra.nsx = c->nsx.p;
+    ra.nsx = L.nsx;
     ra.act_cnt = c->has_act ? c->act_cnt.p : (c->order_ready ? c->ocnt.p : nullptr);
     ra.act_off = c->has_act ? c->act_off.p : (c->order_ready ? c->ooff.p : nullptr);
-    ra.nrows = c->nrows;
+    ra.nrows = L.nrows;
     // chunk_model > 0: at most that many i-particles per item (default 256)
     ra.stage_ops = c->tuning.chunk_model > 0 ? (float)c->tuning.chunk_model : -1.0f;
     ra.ipl_max = c->tuning.ipl_max >= 1 && c->tuning.ipl_max <= kIplMax ? c->tuning.ipl_max : kIplMax;
@@ -1241,7 +1352,7 @@ static RowArgs row_args(pp2p_ctx *c)
 static ForceVariant variant(pp2p_ctx *c, bool queue)
 {
     ForceVariant v;
-    v.law = c->params.law == PP2P_LAW_POLY ? LAW_POLY : LAW_LOGTAB;
+    v.law = c->params.law == PP2P_LAW_POLY ? LAW_POLY : c->params.law == PP2P_LAW_LINEAR ? LAW_LINEAR : LAW_LOGTAB;
     v.soft = c->params.softening == PP2P_SOFT_PLUMMER ? SOFT_PLUMMER : SOFT_REF;
     v.fold64 = c->params.frame == PP2P_FRAME_LEAF;
     v.indirect = c->has_act || c->order_ready;
@@ -1249,88 +1360,115 @@ static ForceVariant variant(pp2p_ctx *c, bool queue)
     return v;
 }
 
-// total persistent waves of the queue form
-static int queue_groups(pp2p_ctx *c)
+// persistent workgroups of the queue form of kernel variant v
+static int queue_groups(pp2p_ctx *c, const ForceVariant &v)
 {
     int per_cu = 0;
-    ForceVariant v = variant(c, true);
-    if (force_occupancy(v, c->law.table_len, &per_cu) != hipSuccess || per_cu <= 0)
-        per_cu = 2;
+    const int wpg = force_waves_per_group(v.law);
+    if (force_occupancy(v, c->table_f4, &per_cu) != hipSuccess || per_cu <= 0)
+        per_cu = std::max(1, 16 / wpg);
     if (c->tuning.waves_per_cu > 0)
-        per_cu = std::max(1, std::min(per_cu, (c->tuning.waves_per_cu + kWavesPerGroup - 1) / kWavesPerGroup));
+        per_cu = std::max(1, std::min(per_cu, (c->tuning.waves_per_cu + wpg - 1) / wpg));
     return per_cu * c->num_cu;
 }
 
-// Work items of the current list: every CSR row cut into i-chunks and source
-// ranges of about `split` pair-interactions.  Two copies: row order (GRID and
-// PER_TARGET forms, h_item_off per row) and sorted by cost, largest first
-// (QUEUE form).
-static int build_items(pp2p_ctx *c)
+static int queue_groups(pp2p_ctx *c) { return queue_groups(c, variant(c, true)); }
+
+static int queue_waves(pp2p_ctx *c)
 {
-    RowArgs ra = row_args(c);
-    const int nr = c->nrows;
-    HIPCHK(c->row_cost.reserve(std::max(nr, 1)));
-    HIPCHK(c->total.reserve(1));
-    HIPCHK(c->item_count.reserve(nr + 1));
-    HIPCHK(c->item_off.reserve(nr + 1));
-    c->nitems = 0;
-    c->h_item_off.assign(1, 0);
-    c->list_pairs = 0;
+    const ForceVariant v = variant(c, true);
+    return queue_groups(c, v) * force_waves_per_group(v.law);
+}
+
+// Work items of a list: every CSR row cut into i-chunks and source ranges of
+// about `split` pair-interactions.  Two copies: row order (GRID and
+// PER_TARGET forms; h_item_off, when given, receives the items per row) and
+// sorted by cost, largest first (QUEUE form).  Two host synchronisations
+// (the pair total, the item count).
+static int items_build(pp2p_ctx *c, Space &sp, ListView &L, std::vector<int> *h_item_off)
+{
+    const int nr = L.nrows;
+    unsigned long long *row_cost = nullptr, *total = nullptr;
+    int *item_count = nullptr, *item_off = nullptr;
+    int st;
+    if ((st = sp.get(c->row_cost, nr, &row_cost)) || (st = sp.get(c->total, 1, &total)) ||
+        (st = sp.get(c->item_count, (size_t)nr + 1, &item_count)) ||
+        (st = sp.get(c->item_off, (size_t)nr + 1, &item_off)))
+        return st;
+    L.nitems = 0;
+    L.pairs = 0;
+    if (h_item_off)
+        h_item_off->assign(1, 0);
     if (nr == 0)
         return PP2P_OK;
-    HIPCHK(c->nsx.reserve(c->ntask + 1));
-    ra.nsx = c->nsx.p;
-    size_t tb0 = 0, tb1 = 0, tb2 = 0;
-    HIPCHK(src_prefix(c->src.p, c->rng.p, c->ntask, c->nsx.p, nullptr, &tb0, c->stream));
-    HIPCHK(rows_cost(ra, c->row_cost.p, c->total.p, nullptr, &tb1, c->stream));
-    HIPCHK(items_count(ra, 0, c->item_count.p, c->item_off.p, nullptr, &tb2, c->stream));
-    int st = ensure_temp(c, std::max(tb0, std::max(tb1, tb2)));
-    if (st)
+    if ((st = sp.get(c->nsx, (size_t)L.ntask + 1, &L.nsx)))
         return st;
-    tb0 = tb1 = tb2 = c->temp.cap;
-    HIPCHK(src_prefix(c->src.p, c->rng.p, c->ntask, c->nsx.p, c->temp.p, &tb0, c->stream));
-    HIPCHK(rows_cost(ra, c->row_cost.p, c->total.p, c->temp.p, &tb1, c->stream));
-    unsigned long long total = 0;
-    HIPCHK(hipMemcpyAsync(&total, c->total.p, sizeof(total), hipMemcpyDeviceToHost, c->stream));
+    RowArgs ra = row_args(c, L);
+    size_t tb0 = 0, tb1 = 0, tb2 = 0;
+    HIPCHK(src_prefix(L.src, c->rng.p, L.ntask, L.nsx, nullptr, &tb0, c->stream));
+    HIPCHK(rows_cost(ra, row_cost, total, nullptr, &tb1, c->stream));
+    HIPCHK(items_count(ra, 0, item_count, item_off, nullptr, &tb2, c->stream));
+    void *tmp = nullptr;
+    size_t tlen = 0;
+    if ((st = sp.temp(std::max(tb0, std::max(tb1, tb2)), &tmp, &tlen)))
+        return st;
+    tb0 = tb1 = tb2 = tlen;
+    HIPCHK(src_prefix(L.src, c->rng.p, L.ntask, L.nsx, tmp, &tb0, c->stream));
+    HIPCHK(rows_cost(ra, row_cost, total, tmp, &tb1, c->stream));
+    unsigned long long pairs = 0;
+    HIPCHK(hipMemcpyAsync(&pairs, total, sizeof(pairs), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    c->list_pairs = total;
+    L.pairs = pairs;
 
     long long split;
     if (c->tuning.split_pairs > 0) {
         split = c->tuning.split_pairs;
     } else {
         // about 8 items per persistent wave, within [2^14, 2^22] pairs
-        const long long waves = (long long)queue_groups(c) * kWavesPerGroup;
-        split = (long long)(total / (unsigned long long)std::max(1ll, 8 * waves));
+        const long long waves = (long long)queue_waves(c);
+        split = (long long)(pairs / (unsigned long long)std::max(1ll, 8 * waves));
         split = std::max(split, 1ll << 14);
         split = std::min(split, 1ll << 22);
     }
-    HIPCHK(launch_fill_i32(c->item_count.p + nr, 1, 0, c->stream));
-    HIPCHK(items_count(ra, split, c->item_count.p, c->item_off.p, c->temp.p, &tb2, c->stream));
+    HIPCHK(launch_fill_i32(item_count + nr, 1, 0, c->stream));
+    HIPCHK(items_count(ra, split, item_count, item_off, tmp, &tb2, c->stream));
     int nitems = 0;
-    HIPCHK(hipMemcpyAsync(&nitems, c->item_off.p + nr, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(&nitems, item_off + nr, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    HIPCHK(c->items_rows.reserve(std::max(nitems, 1)));
-    HIPCHK(c->items_q.reserve(std::max(nitems, 1)));
-    HIPCHK(c->cost.reserve(std::max(nitems, 1)));
-    HIPCHK(c->cost_sorted.reserve(std::max(nitems, 1)));
-    HIPCHK(c->idx.reserve(std::max(nitems, 1)));
-    HIPCHK(c->idx_sorted.reserve(std::max(nitems, 1)));
-    HIPCHK(items_emit(ra, split, c->item_off.p, c->items_rows.p, c->cost.p, c->stream));
-    size_t tb3 = 0;
-    HIPCHK(items_sort(c->cost.p, c->cost_sorted.p, c->idx.p, c->idx_sorted.p, c->items_rows.p, c->items_q.p, nitems,
-                      nullptr, &tb3, c->stream));
-    if ((st = ensure_temp(c, tb3)))
+    unsigned *cost = nullptr, *cost_sorted = nullptr;
+    int *idx = nullptr, *idx_sorted = nullptr;
+    if ((st = sp.get(c->items_rows, nitems, &L.items_rows)) || (st = sp.get(c->items_q, nitems, &L.items_q)) ||
+        (st = sp.get(c->cost, nitems, &cost)) || (st = sp.get(c->cost_sorted, nitems, &cost_sorted)) ||
+        (st = sp.get(c->idx, nitems, &idx)) || (st = sp.get(c->idx_sorted, nitems, &idx_sorted)))
         return st;
-    tb3 = c->temp.cap;
-    HIPCHK(items_sort(c->cost.p, c->cost_sorted.p, c->idx.p, c->idx_sorted.p, c->items_rows.p, c->items_q.p, nitems,
-                      c->temp.p, &tb3, c->stream));
-    c->h_item_off.resize(nr + 1);
-    HIPCHK(hipMemcpyAsync(c->h_item_off.data(), c->item_off.p, sizeof(int) * (nr + 1), hipMemcpyDeviceToHost,
-                          c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    c->nitems = nitems;
-    c->split = split;
+    HIPCHK(items_emit(ra, split, item_off, L.items_rows, cost, c->stream));
+    size_t tb3 = 0;
+    HIPCHK(items_sort(cost, cost_sorted, idx, idx_sorted, L.items_rows, L.items_q, nitems, nullptr, &tb3,
+                      c->stream));
+    if ((st = sp.temp(tb3, &tmp, &tb3)))
+        return st;
+    HIPCHK(items_sort(cost, cost_sorted, idx, idx_sorted, L.items_rows, L.items_q, nitems, tmp, &tb3, c->stream));
+    if (h_item_off) {
+        h_item_off->resize(nr + 1);
+        HIPCHK(hipMemcpyAsync(h_item_off->data(), item_off, sizeof(int) * (nr + 1), hipMemcpyDeviceToHost,
+                              c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    L.nitems = nitems;
+    L.split = split;
+    return PP2P_OK;
+}
+
+static int build_items(pp2p_ctx *c)
+{
+    Space sp(c);
+    ListView L = ctx_view(c);
+    int st = items_build(c, sp, L, &c->h_item_off);
+    if (st)
+        return st;
+    c->list_pairs = L.pairs;
+    c->nitems = L.nitems;
+    c->split = L.split;
     return PP2P_OK;
 }
 
@@ -1379,11 +1517,16 @@ static ForceArgs force_args(pp2p_ctx *c, bool queue)
     a.ilist = c->has_act ? c->ilist.p : (c->order_ready ? c->olist.p : nullptr);
     a.acc = c->acc.p;
     a.npart = c->npart;
-    a.box = c->params.box / c->law.rc;
-    a.out_scale = c->params.mass / (c->law.rc * c->law.rc);
+    // the leaf frames hold positions in units of rc; the absolute FP32 frame
+    // keeps the input units (the reference's float coordinates), so its sums
+    // carry one more 1 / rc and its cull distance is in input units
+    const bool abs = c->params.frame == PP2P_FRAME_ABS32;
+    const double rc = c->law.rc;
+    a.box = c->params.box / rc;
+    a.out_scale = c->params.mass / (rc * rc) / (abs ? rc : 1.0);
     a.table = c->table.p;
-    a.table_len = c->law.table_len;
-    a.cull = c->tuning.no_cull ? 0 : 1;
+    a.table_len = c->table_f4;
+    a.cull_lim = c->tuning.no_cull ? 3.0e38f : (float)(1.0009765625 * (abs ? rc * rc : 1.0));
     return a;
 }
 
@@ -1393,14 +1536,15 @@ static int enqueue_compute(pp2p_ctx *c, int exec)
     const bool queue = exec == PP2P_EXEC_QUEUE;
     ForceArgs a = force_args(c, queue);
     const ForceVariant v = variant(c, queue);
+    const int wpg = force_waves_per_group(v.law);
     if (a.nitems == 0)
         return PP2P_OK;
     if (queue) {
         HIPCHK(launch_reset_counter(c->counter.p, c->stream));
-        const int groups = std::min(queue_groups(c), (a.nitems + kWavesPerGroup - 1) / kWavesPerGroup);
+        const int groups = std::min(queue_groups(c, v), (a.nitems + wpg - 1) / wpg);
         HIPCHK(launch_force(a, c->law.lc, v, groups, c->stream));
     } else if (exec == PP2P_EXEC_GRID) {
-        HIPCHK(launch_force(a, c->law.lc, v, (a.nitems + kWavesPerGroup - 1) / kWavesPerGroup, c->stream));
+        HIPCHK(launch_force(a, c->law.lc, v, (a.nitems + wpg - 1) / wpg, c->stream));
     } else {
         for (int r = 0; r < c->nrows; ++r) {
             const int b = c->h_item_off[r], e = c->h_item_off[r + 1];
@@ -1408,7 +1552,7 @@ static int enqueue_compute(pp2p_ctx *c, int exec)
                 continue;
             a.item_base = b;
             a.nitems = e;
-            HIPCHK(launch_force(a, c->law.lc, v, (e - b + kWavesPerGroup - 1) / kWavesPerGroup, c->stream));
+            HIPCHK(launch_force(a, c->law.lc, v, (e - b + wpg - 1) / wpg, c->stream));
         }
     }
     return PP2P_OK;
@@ -1598,10 +1742,13 @@ int pp2p_count_inside(pp2p_ctx *c, long long *n_inside)
     HIPCHK(hipMemcpyAsync(c->pair_count.p, &zero, sizeof(zero), hipMemcpyHostToDevice, c->stream));
     ForceArgs a = force_args(c, true);
     a.count = c->pair_count.p;
+    a.table_len = 0;                       // the counting pass reads no table
     if (a.nitems > 0) {
         HIPCHK(launch_reset_counter(c->counter.p, c->stream));
-        const int groups = std::min(queue_groups(c), (a.nitems + kWavesPerGroup - 1) / kWavesPerGroup);
-        HIPCHK(launch_count(a, c->law.lc, variant(c, true), groups, c->stream));
+        ForceVariant v = variant(c, true);
+        v.law = LAW_LOGTAB;                // the counting kernels have the table law's workgroup shape
+        const int groups = std::min(queue_groups(c, v), (a.nitems + kWavesPerGroup - 1) / kWavesPerGroup);
+        HIPCHK(launch_count(a, c->law.lc, v, groups, c->stream));
     }
     HIPCHK(hipMemcpyAsync(&total, c->pair_count.p, sizeof(total), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -1691,6 +1838,18 @@ int pp2p_law_eval(const pp2p_params *p, int segs, long long n, const float *s, f
     const char *err = nullptr;
     if (!law_setup(p->softening, p->eps, p->rs, p->ucut, segs ? segs : 32, &ls, &err))
         return fail(PP2P_ERR_ARG, err);
+    if (p->law == PP2P_LAW_LINEAR) {
+        if (!ls.lin_ok)
+            return fail(PP2P_ERR_ARG, "force law: eps / rc too small for the linear law");
+        for (long long k = 0; k < n; ++k) {
+            // the kernel's s lies in [smin, 1]
+            const float sc = std::fmin(std::fmax(s[k], ls.lc.smin), 1.0f);
+            F[k] = law_linear_eval(ls, sc);
+        }
+        if (table_len)
+            *table_len = (int)(ls.lin.size() / 2);
+        return PP2P_OK;
+    }
     if (!ls.table_finite)
         return fail(PP2P_ERR_ARG, "force law: eps / rc too small for the table law (its cubic coefficients "
                                   "overflow FP32 below eps / rc ~ 6e-5, its octaves wrap below 2^-15.5)");

@@ -76,11 +76,18 @@ __device__ __forceinline__ float wave_max(float v)
 // The clamps compare bit patterns as unsigned integers, which orders
 // non-negative floats like their values (integer VALU ops issue at twice the
 // FP32 rate); any NaN pattern is above the pattern of 1.0, so the result lies
-// in [smin, 1] for every input.
-template <int SOFT>
+// in [smin, 1] for every input.  SCALE: the differences are in input length
+// units (absolute FP32 frame) and r^2 is scaled by 1 / rc^2 first.
+template <int SOFT, bool SCALE = false>
 __device__ __forceinline__ float soft_s(float dx, float dy, float dz, const LawConst &lc)
 {
     constexpr unsigned kOne = 0x3f800000u;
+    if (SCALE) {
+        const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+        if (SOFT == SOFT_REF)
+            return __uint_as_float(min(max(__float_as_uint(r2 * lc.abs_s2), __float_as_uint(lc.smin)), kOne));
+        return __uint_as_float(min(__float_as_uint(fmaf(r2, lc.abs_s2, lc.smin)), kOne));
+    }
     if (SOFT == SOFT_REF) {                                 // max(r, eps), clamp
         const unsigned r2 = __float_as_uint(fmaf(dz, dz, fmaf(dy, dy, dx * dx)));
         return __uint_as_float(min(max(r2, __float_as_uint(lc.smin)), kOne));
@@ -111,9 +118,21 @@ __device__ __forceinline__ float logtab_eval(float4 c, float t)
     return fmaf(fmaf(fmaf(c.w, t, c.z), t, c.y), t, c.x);
 }
 
+// LINEAR law.  F(s) = a + b s on 512 segments per binary octave: the low 5
+// exponent bits and the top 9 mantissa bits of s, shifted into place, are the
+// LDS byte address of the segment's (a, b) pair (one shift and one AND), and
+// one FMA evaluates it.  s lies in [smin, 1] and the host places every
+// octave from smin's up to the entry of s = 1 (zero) at those addresses, so
+// the address stays inside the table.
+__device__ __forceinline__ float2 linear_segment(float s, const char *__restrict__ lds)
+{
+    const unsigned off = (__float_as_uint(s) >> (kLinSegShift - 3)) & 0x1fff8u;
+    return *reinterpret_cast<const float2 *>(lds + off);
+}
+
 // One pair, POLY law: F = exp(-u^2) h(r) / r^3 with h(r) = erfcx(u) + (2/sqrt(pi)) u
 // as a degree-12 polynomial in r = sqrt(s) in [0, 1]; explicit cutoff select.
-template <int SOFT>
+template <int SOFT, bool SCALE>
 __device__ __forceinline__ void pair_poly(float xi, float yi, float zi, float4 pj,
                                           const LawConst &lc,
                                           float &ax, float &ay, float &az)
@@ -121,7 +140,7 @@ __device__ __forceinline__ void pair_poly(float xi, float yi, float zi, float4 p
     const float dx = pj.x - xi;
     const float dy = pj.y - yi;
     const float dz = pj.z - zi;
-    const float s = soft_s<SOFT>(dx, dy, dz, lc);
+    const float s = soft_s<SOFT, SCALE>(dx, dy, dz, lc);
     const float rinv = __builtin_amdgcn_rsqf(s);
     const float r = s * rinv;
     const float e = __builtin_amdgcn_exp2f(s * lc.k2);
@@ -197,7 +216,7 @@ __device__ __forceinline__ void process_item(const ForceArgs &a, const LawConst 
     const float cx = uniform(0.5f * (bxl + bxh)), hx = uniform(0.5f * (bxh - bxl));
     const float cy = uniform(0.5f * (byl + byh)), hy = uniform(0.5f * (byh - byl));
     const float cz = uniform(0.5f * (bzl + bzh)), hz = uniform(0.5f * (bzh - bzl));
-    const float cull_lim = a.cull ? 1.0009765625f : 3.0e38f;   // s > 1 + 2^-10: no pair can reach
+    const float cull_lim = a.cull_lim;              // squared distance to the item box beyond which no pair reaches
 
     const double4 rt = a.ref[it.tgt];
     double sx[IPL], sy[IPL], sz[IPL];
@@ -215,13 +234,31 @@ __device__ __forceinline__ void process_item(const ForceArgs &a, const LawConst 
     // the IPL pairs of one j: all table addresses first, then the gathers,
     // then the polynomials, so that the gathers of a step are in flight together
     auto pairs_of = [&](const float4 &pj) {
+        if (!COUNT && LAW == LAW_LINEAR) {
+            float dx[IPL], dy[IPL], dz[IPL], sv[IPL];
+            float2 c[IPL];
+#pragma unroll
+            for (int k = 0; k < IPL; ++k) {
+                dx[k] = pj.x - xi[k]; dy[k] = pj.y - yi[k]; dz[k] = pj.z - zi[k];
+                sv[k] = soft_s<SOFT, !FOLD64>(dx[k], dy[k], dz[k], lc);
+                c[k] = linear_segment(sv[k], lds);
+            }
+#pragma unroll
+            for (int k = 0; k < IPL; ++k) {
+                const float f = fmaf(c[k].y, sv[k], c[k].x);
+                fx[k] = fmaf(dx[k], f, fx[k]);
+                fy[k] = fmaf(dy[k], f, fy[k]);
+                fz[k] = fmaf(dz[k], f, fz[k]);
+            }
+            return;
+        }
         if (!COUNT && LAW == LAW_LOGTAB) {
             float dx[IPL], dy[IPL], dz[IPL], t[IPL];
             float4 c[IPL];
 #pragma unroll
             for (int k = 0; k < IPL; ++k) {
                 dx[k] = pj.x - xi[k]; dy[k] = pj.y - yi[k]; dz[k] = pj.z - zi[k];
-                c[k] = logtab_segment(soft_s<SOFT>(dx[k], dy[k], dz[k], lc), lds, seg_mask, tab_shift, t[k]);
+                c[k] = logtab_segment(soft_s<SOFT, !FOLD64>(dx[k], dy[k], dz[k], lc), lds, seg_mask, tab_shift, t[k]);
             }
 #pragma unroll
             for (int k = 0; k < IPL; ++k) {
@@ -236,9 +273,9 @@ __device__ __forceinline__ void process_item(const ForceArgs &a, const LawConst 
         for (int k = 0; k < IPL; ++k) {
             if (COUNT) {
                 const float dx = pj.x - xi[k], dy = pj.y - yi[k], dz = pj.z - zi[k];
-                n_in += valid[k] && soft_s<SOFT>(dx, dy, dz, lc) < 1.0f;
+                n_in += valid[k] && soft_s<SOFT, !FOLD64>(dx, dy, dz, lc) < 1.0f;
             } else {
-                pair_poly<SOFT>(xi[k], yi[k], zi[k], pj, lc, fx[k], fy[k], fz[k]);
+                pair_poly<SOFT, !FOLD64>(xi[k], yi[k], zi[k], pj, lc, fx[k], fy[k], fz[k]);
             }
         }
     };
@@ -415,26 +452,30 @@ __device__ __forceinline__ void process_item(const ForceArgs &a, const LawConst 
 // Persistent (QUEUE) or one-wave-per-item kernel.  The item's IPL selects
 // one of three specialisations; the branch is uniform per wave.
 template <int LAW, int SOFT, bool FOLD64, bool INDIRECT, bool QUEUE, bool COUNT = false>
-__global__ __launch_bounds__(kWave * kWavesPerGroup)
+__global__ __launch_bounds__(kWave * force_waves_per_group(LAW))
 __attribute__((amdgpu_waves_per_eu(PP2P_WAVES_PER_EU)))
 void p2p_force(ForceArgs a, LawConst lc)
 {
-    __shared__ float4 s_lds[kLdsBytes / 16];                // the workgroup's whole LDS (layout: p2p_device.h)
+    constexpr bool kLin = LAW == LAW_LINEAR;
+    constexpr int kWpg = force_waves_per_group(LAW);
+    // the workgroup's whole LDS (layouts: p2p_device.h)
+    __shared__ float4 s_lds[(kLin ? kLinLdsBytes : kLdsBytes) / 16];
     char *lds = reinterpret_cast<char *>(s_lds);
 
     const int wv = threadIdx.x / kWave;
     const int lane = threadIdx.x % kWave;
 
-    if (LAW == LAW_LOGTAB) {
-        float4 *tab = reinterpret_cast<float4 *>(lds + lds_table_off(lc.seg_shift));
+    if (LAW == LAW_LOGTAB || kLin) {
+        float4 *tab = reinterpret_cast<float4 *>(lds + (kLin ? lc.lin_lo : lds_table_off(lc.seg_shift)));
         for (int k = threadIdx.x; k < a.table_len; k += blockDim.x)
             tab[k] = a.table[k];
         __syncthreads();
     }
 
-    float4 *tile = reinterpret_cast<float4 *>(lds + kLdsTileOff) + wv * kTileCap;
-    SrcDir &dir = *reinterpret_cast<SrcDir *>(lds + kLdsDirOff + wv * kLdsDirBytes);
-    int item_id = QUEUE ? 0 : a.item_base + blockIdx.x * kWavesPerGroup + wv;
+    float4 *tile = reinterpret_cast<float4 *>(lds + (kLin ? kLinTileOff : kLdsTileOff)) + wv * kTileCap;
+    SrcDir &dir = *reinterpret_cast<SrcDir *>(lds + (kLin ? kLinDirOff + wv * kLinDirStride
+                                                         : kLdsDirOff + wv * kLdsDirBytes));
+    int item_id = QUEUE ? 0 : a.item_base + blockIdx.x * kWpg + wv;
     unsigned long long n_in = 0;
 
     for (;;) {
@@ -475,7 +516,7 @@ template <int LAW, int SOFT, bool FOLD64, bool INDIRECT, bool QUEUE>
 static hipError_t launch_one(const ForceArgs &a, const LawConst &lc, int grid, hipStream_t st)
 {
     hipLaunchKernelGGL((p2p_force<LAW, SOFT, FOLD64, INDIRECT, QUEUE>), dim3(grid),
-                       dim3(kWave * kWavesPerGroup), 0, st, a, lc);
+                       dim3(kWave * force_waves_per_group(LAW)), 0, st, a, lc);
     return hipGetLastError();
 }
 
@@ -502,6 +543,9 @@ static hipError_t launch_f(const ForceArgs &a, const LawConst &lc, bool fold64, 
 
 hipError_t launch_force(const ForceArgs &a, const LawConst &lc, const ForceVariant &v, int grid, hipStream_t st)
 {
+    if (v.law == LAW_LINEAR)
+        return v.soft == SOFT_REF ? launch_f<LAW_LINEAR, SOFT_REF>(a, lc, v.fold64, v.indirect, v.queue, grid, st)
+                                  : launch_f<LAW_LINEAR, SOFT_PLUMMER>(a, lc, v.fold64, v.indirect, v.queue, grid, st);
     if (v.law == LAW_LOGTAB)
         return v.soft == SOFT_REF ? launch_f<LAW_LOGTAB, SOFT_REF>(a, lc, v.fold64, v.indirect, v.queue, grid, st)
                                   : launch_f<LAW_LOGTAB, SOFT_PLUMMER>(a, lc, v.fold64, v.indirect, v.queue, grid, st);
@@ -512,6 +556,9 @@ hipError_t launch_force(const ForceArgs &a, const LawConst &lc, const ForceVaria
 hipError_t force_occupancy(const ForceVariant &v, int table_len, int *groups_per_cu)
 {
     (void)table_len;                                        // the table lives in the static LDS array
+    if (v.law == LAW_LINEAR)
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(groups_per_cu,
+                (const void *)p2p_force<LAW_LINEAR, SOFT_REF, true, false, true>, kWave * kLinWaves, 0);
     if (v.law == LAW_LOGTAB)
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(groups_per_cu,
                 (const void *)p2p_force<LAW_LOGTAB, SOFT_REF, true, false, true>, kWave * kWavesPerGroup, 0);
@@ -519,23 +566,29 @@ hipError_t force_occupancy(const ForceVariant &v, int table_len, int *groups_per
             (const void *)p2p_force<LAW_POLY, SOFT_REF, true, false, true>, kWave * kWavesPerGroup, 0);
 }
 
+template <int SOFT, bool FOLD64>
+static void launch_count_one(const ForceArgs &a, const LawConst &lc, bool indirect, int grid, hipStream_t st)
+{
+    if (indirect)
+        hipLaunchKernelGGL((p2p_force<LAW_LOGTAB, SOFT, FOLD64, true, true, true>), dim3(grid),
+                           dim3(kWave * kWavesPerGroup), 0, st, a, lc);
+    else
+        hipLaunchKernelGGL((p2p_force<LAW_LOGTAB, SOFT, FOLD64, false, true, true>), dim3(grid),
+                           dim3(kWave * kWavesPerGroup), 0, st, a, lc);
+}
+
 hipError_t launch_count(const ForceArgs &a, const LawConst &lc, const ForceVariant &v, int grid, hipStream_t st)
 {
-    const size_t lds = 0;
     if (v.soft == SOFT_REF) {
-        if (v.indirect)
-            hipLaunchKernelGGL((p2p_force<LAW_LOGTAB, SOFT_REF, true, true, true, true>), dim3(grid),
-                               dim3(kWave * kWavesPerGroup), lds, st, a, lc);
+        if (v.fold64)
+            launch_count_one<SOFT_REF, true>(a, lc, v.indirect, grid, st);
         else
-            hipLaunchKernelGGL((p2p_force<LAW_LOGTAB, SOFT_REF, true, false, true, true>), dim3(grid),
-                               dim3(kWave * kWavesPerGroup), lds, st, a, lc);
+            launch_count_one<SOFT_REF, false>(a, lc, v.indirect, grid, st);
     } else {
-        if (v.indirect)
-            hipLaunchKernelGGL((p2p_force<LAW_LOGTAB, SOFT_PLUMMER, true, true, true, true>), dim3(grid),
-                               dim3(kWave * kWavesPerGroup), lds, st, a, lc);
+        if (v.fold64)
+            launch_count_one<SOFT_PLUMMER, true>(a, lc, v.indirect, grid, st);
         else
-            hipLaunchKernelGGL((p2p_force<LAW_LOGTAB, SOFT_PLUMMER, true, false, true, true>), dim3(grid),
-                               dim3(kWave * kWavesPerGroup), lds, st, a, lc);
+            launch_count_one<SOFT_PLUMMER, false>(a, lc, v.indirect, grid, st);
     }
     return hipGetLastError();
 }

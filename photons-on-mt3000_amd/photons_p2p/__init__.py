@@ -27,7 +27,7 @@ HOST_LIB = os.path.join(LIB_DIR, "libphotons_host.so")
 # pp2p_status
 OK, ERR_ARG, ERR_DEVICE, ERR_ALLOC, ERR_STATE, ERR_NOGPU = range(6)
 SOFT_REF, SOFT_PLUMMER = 0, 1
-LAW_TABLE, LAW_POLY = 0, 1
+LAW_TABLE, LAW_POLY, LAW_LINEAR = 0, 1, 2
 FRAME_LEAF, FRAME_ABS32 = 0, 1
 EXEC_QUEUE, EXEC_GRID, EXEC_PER_TARGET = 0, 1, 2
 

@@ -7,7 +7,8 @@ fastest.  The knobs are the ones pp2p_tuning exposes:
   split_pairs  size of a queue item (pair-interactions) = the j-range a wave
                owns before it fetches the next item
   waves_per_cu persistent waves per CU of the queue form (0 = occupancy max)
-  law          LOGTAB table or the table-free POLY law
+  law          the cubic LOGTAB table, the linear table (LINEAR) or the
+               table-free POLY law
 Tuned values are kept in photons_p2p/tuned.json, keyed by a signature of
 the workload shape (particle, leaf, task and pair counts in half-octave
 buckets); `lookup` returns the entry for a shape, `tune` measures a new one.
@@ -18,7 +19,7 @@ import json
 import os
 import time
 
-from . import Context, EXEC_QUEUE, ForceLaw, LAW_POLY, LAW_TABLE
+from . import Context, EXEC_QUEUE, ForceLaw, LAW_LINEAR, LAW_POLY, LAW_TABLE, P2PError
 
 TUNED = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned.json")
 
@@ -26,7 +27,7 @@ GRID = {
     "ipl_max": [1, 2, 3],
     "split_pairs": [0, 1 << 16, 1 << 18, 1 << 20],
     "waves_per_cu": [0, 12, 8],
-    "law": [LAW_TABLE, LAW_POLY],
+    "law": [LAW_TABLE, LAW_LINEAR, LAW_POLY],
 }
 
 
@@ -63,7 +64,11 @@ def tune(ctx: Context, law: ForceLaw, steps: int = 3, grid: dict | None = None, 
             if v == best[knob] or time.time() - t_start > budget_s:
                 continue
             cfg = {**best, knob: v}
-            set_cfg(cfg)
+            try:
+                set_cfg(cfg)
+            except P2PError:                        # e.g. a law whose range excludes this softening
+                set_cfg(best)
+                continue
             ms = _time(ctx, steps)
             trials.append({**cfg, "ms": ms})
             if log:

@@ -81,6 +81,41 @@ def test_device_table_law_accuracy(softening, eps_div):
     assert rel.max() < 5e-7 and np.median(rel) < 5e-8
 
 
+@pytest.mark.parametrize("softening,eps_div", [(pp.SOFT_PLUMMER, 40.0), (pp.SOFT_REF, 1 / 0.03), (pp.SOFT_PLUMMER, 2.0)])
+def test_device_linear_law_accuracy(softening, eps_div):
+    """The linear law (a + b s on 512 segments per octave, the line through
+    F at each segment's two Chebyshev points) against F(s) from scipy's erfc:
+    the interpolation error of a line is about (F''/16) w^2, 0.23 (w/s)^2 for
+    the s^-3/2 power law (w/s <= 2^-9: 9e-7) and larger just below the cutoff,
+    where g(u) falls steeply."""
+    d = 400.0 / 256
+    law = pp.ForceLaw(eps=d / eps_div, rs=1.25 * d, softening=softening, law=pp.LAW_LINEAR)
+    smin = (law.eps / law.rc) ** 2
+    s = np.geomspace(smin, 1.0 - 1e-6, 300001).astype(np.float32)
+    s = np.maximum(s, np.float32(smin))
+    F, n = pp.law_eval(law, s)
+    e_lo = int(np.floor(np.log2(np.float32(smin))))
+    assert n == -e_lo * 512 + 1                       # octaves of smin up to 1, and the entry of s = 1
+    rel = np.abs(F / F_exact(s.astype(np.float64), law.ucut) - 1)
+    low = s < 0.25
+    assert rel.max() < 8e-6 and rel[low].max() < 2e-6 and np.median(rel) < 5e-7
+    F1, _ = pp.law_eval(law, np.array([1.0, 1.5], np.float32))
+    assert F1[0] == 0 and F1[1] == 0                  # at and beyond the cutoff
+
+
+def test_linear_law_range():
+    """The linear law's octaves of s must fit the LDS: eps / rc >= 2^-12.5
+    (smin >= 2^-25); below it the law is refused, the table law still works."""
+    rs = 1.0
+    rc = 2 * rs * 2.8125
+    with pytest.raises(pp.P2PError):
+        pp.law_eval(pp.ForceLaw(eps=2.0 ** -13 * rc, rs=rs, law=pp.LAW_LINEAR), np.ones(1, np.float32))
+    pp.law_eval(pp.ForceLaw(eps=2.0 ** -13 * rc, rs=rs), np.ones(1, np.float32))
+    F, _ = pp.law_eval(pp.ForceLaw(eps=2.0 ** -12.4 * rc, rs=rs, law=pp.LAW_LINEAR),
+                       np.geomspace(2.0 ** -24.8, 0.999, 1000).astype(np.float32))
+    assert np.all(np.isfinite(F)) and np.all(F > 0)
+
+
 def test_device_table_cutoff_and_continuity():
     law = pp.ForceLaw(eps=0.04, rs=2.0)
     F, _ = pp.law_eval(law, np.array([1.0, 1.25, 2.0, 0.9999999], np.float32))

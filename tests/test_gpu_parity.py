@@ -56,13 +56,25 @@ def test_uneven_plummer_periodic(gpu_ctx, exec_form):
     assert p99 <= P99_TOL and mx <= MAX_TOL
 
 
-@pytest.mark.parametrize("law", [pp.LAW_TABLE, pp.LAW_POLY])
+@pytest.mark.parametrize("law", [pp.LAW_TABLE, pp.LAW_POLY, pp.LAW_LINEAR])
 def test_ref_softening_laws(gpu_ctx, law):
     w = _small_uneven(seed=12, softening=0)
     ref = _oracle(w)
     a = _run(gpu_ctx, w, law=law)
     med, p99, mx = pyoracle.rel_err(a, ref)
     print(f"law {law}: rel err median {med:.3e} p99 {p99:.3e} max {mx:.3e}")
+    assert p99 <= P99_TOL and mx <= MAX_TOL
+
+
+@pytest.mark.parametrize("exec_form", [pp.EXEC_QUEUE, pp.EXEC_GRID, pp.EXEC_PER_TARGET])
+def test_linear_law_plummer_periodic(gpu_ctx, exec_form):
+    """The linear law (one workgroup of 16 waves per CU, 160 KiB of LDS) in
+    every execution form, periodic, Plummer softening."""
+    w = _small_uneven(seed=18)
+    ref = _oracle(w)
+    a = _run(gpu_ctx, w, exec_form, law=pp.LAW_LINEAR)
+    med, p99, mx = pyoracle.rel_err(a, ref)
+    print(f"linear law: rel err median {med:.3e} p99 {p99:.3e} max {mx:.3e}")
     assert p99 <= P99_TOL and mx <= MAX_TOL
 
 
@@ -103,15 +115,28 @@ def test_graph_replay_matches_eager(gpu_ctx):
 
 
 def test_abs32_frame_matches_reference_form(gpu_ctx):
-    """FP32 absolute coordinates: compare with the FP32 restatement of the
-    reference kernel on the same float32 positions (open boundaries)."""
+    """FP32 absolute coordinates (the reference's scheme, open boundaries):
+    the device in that frame, and the FP32 restatement of the reference
+    kernel (oracle_p2p_ref32: P2P_kernel / P2P_core_1, 512-node table and
+    4th-order Taylor) on the same float32 positions, are both compared with
+    the FP64 oracle.  Coordinate rounding at |x| ~ 50 dominates both, so the
+    device must sit at the restatement's error level (p99 within 2x of it,
+    and <= 1e-4) and agree with the restatement to p99 <= 2e-4."""
     w = _small_uneven(seed=16)
     w.pos = w.pos.astype(np.float32).astype(np.float64)
     a = _run(gpu_ctx, w, frame=pp.FRAME_ABS32, box=0.0)
     ref64 = _oracle(w, box=0.0)
-    med, p99, mx = pyoracle.rel_err(a, ref64)
-    print(f"abs32 vs f64: median {med:.3e} p99 {p99:.3e} max {mx:.3e}")
-    assert p99 <= 1e-4
+    prm = pyoracle.params(w.eps, w.rs, mass=w.mass, softening=w.softening)
+    csr = pyoracle.build_csr(w.task_t, w.task_s, w.nleaf)
+    r32 = pyoracle.p2p_ref32(prm, np.ascontiguousarray(w.pos.T.astype(np.float32)), w.leaf_ip, w.leaf_np,
+                             csr).T.astype(np.float64)
+    d_gpu = pyoracle.rel_err(a, ref64)
+    d_r32 = pyoracle.rel_err(r32, ref64)
+    d_ref = pyoracle.rel_err(a, r32)
+    print(f"abs32: device vs f64 median {d_gpu[0]:.3e} p99 {d_gpu[1]:.3e} max {d_gpu[2]:.3e}; "
+          f"ref32 vs f64 p99 {d_r32[1]:.3e}; device vs ref32 p99 {d_ref[1]:.3e} max {d_ref[2]:.3e}")
+    assert d_gpu[1] <= 1e-4 and d_gpu[1] <= 2.0 * d_r32[1]
+    assert d_ref[1] <= 2e-4
 
 
 def test_i32_positions(gpu_ctx):
