@@ -267,6 +267,25 @@ int pp2p_prepare(pp2p_ctx *ctx);
  * (+=) into the context's accumulator.  Asynchronous. */
 int pp2p_compute(pp2p_ctx *ctx, int exec);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): pp2p_compute(ctx, PP2P_EXEC_QUEUE) over a list given
+ * as device arrays, accumulating (+=) into a device array, with the list's
+ * CSR, work items, sort storage and queue head in the library's static
+ * device scratch array (64 MiB per GPU, held until the process ends)
+ * instead of the context's buffers.  It calls no hipMalloc, hipFree or
+ * allocating library routine.  The particles, law and active mask are the
+ * context's (set beforehand).  task_t, task_s: ntask device ints in
+ * [0, nleaf); acc: 3 planes of npart device doubles (x, y, z).  A list
+ * whose CSR and items need more than the scratch is refused with
+ * PP2P_ERR_ARG (about 1.5 M leaf pairs of leaves of up to 64 particles).
+ * The call synchronises the context stream while it builds the list, then
+ * enqueues the force launch; acc is complete after pp2p_sync(ctx).  Every
+ * call on one GPU shares that GPU's scratch, which keeps the last call's
+ * contents: calls that overlap on one GPU (other contexts or threads)
+ * race. */
+int pp2p_compute_noalloc(pp2p_ctx *ctx, long long ntask, const int *task_t, const int *task_s, double *acc);
+
 /* Zeroes the accumulator (asynchronous). */
 int pp2p_zero_acc(pp2p_ctx *ctx);
 

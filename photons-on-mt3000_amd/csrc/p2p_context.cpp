@@ -17,6 +17,7 @@
 #include "p2p_launch.h"
 #include "p2p_m2l.h"
 #include "p2p_pm.h"
+#include "p2p_scratch.h"
 #include "p2p_tree.h"
 #include "p2p_walk.h"
 
@@ -72,6 +73,7 @@ struct DevBuf {
 struct pp2p_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    void *scratch = nullptr;         // this device's static scratch (pp2p_compute_noalloc)
     hipEvent_t ev_k0 = nullptr, ev_k1 = nullptr, ev_p0 = nullptr, ev_p1 = nullptr;
     int num_cu = 0;
 
@@ -243,6 +245,10 @@ pp2p_ctx *pp2p_create(int device, int *status)
             c = nullptr;
         } else {
             c->num_cu = prop.multiProcessorCount;
+            // once per device, with the device current; a failure only
+            // disables pp2p_compute_noalloc
+            if (scratch_address(device, &c->scratch) != hipSuccess)
+                c->scratch = nullptr;
         }
     }
     if (status)
@@ -1572,6 +1578,52 @@ int pp2p_compute(pp2p_ctx *c, int exec)
     HIPCHK(hipEventRecord(c->ev_k0, c->stream));
     if ((st = enqueue_compute(c, exec)))
         return st;
+    HIPCHK(hipEventRecord(c->ev_k1, c->stream));
+    c->kernel_timed = true;
+    return PP2P_OK;
+}
+
+int pp2p_compute_noalloc(pp2p_ctx *c, long long ntask, const int *task_t, const int *task_s, double *acc)
+{
+    int st = check_ctx(c);
+    if (st)
+        return st;
+    if (!c->have_particles)
+        return fail(PP2P_ERR_STATE, "set_particles before compute_noalloc");
+    if (ntask < 0 || ntask > (1ll << 31) - 1 || (ntask > 0 && (!task_t || !task_s)) || !acc)
+        return fail(PP2P_ERR_ARG, "compute_noalloc: 0 <= ntask < 2^31, device task arrays and a device accumulator");
+    if (!c->scratch)
+        return fail(PP2P_ERR_DEVICE, "compute_noalloc: the scratch array of this device has no address");
+    Space sp(c, static_cast<char *>(c->scratch), kScratchBytes);
+    int *flag = nullptr;
+    unsigned *counter = nullptr;
+    if ((st = sp.carve(sizeof(int), reinterpret_cast<void **>(&flag))) ||
+        (st = sp.carve(sizeof(unsigned), reinterpret_cast<void **>(&counter))))
+        return st;
+    // the scratch keeps the last call's contents: clear the flag first
+    HIPCHK(launch_fill_i32(flag, 1, 0, c->stream));
+    HIPCHK(launch_check_tasks(task_t, task_s, ntask, c->nleaf, flag, c->stream));
+    int bad = 0;
+    HIPCHK(hipMemcpyAsync(&bad, flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (bad)
+        return fail(PP2P_ERR_ARG, "compute_noalloc: a task names a leaf outside [0, nleaf)");
+    ListView L;
+    if ((st = csr_build(c, sp, task_t, task_s, (int)ntask, L)) || (st = items_build(c, sp, L, nullptr)))
+        return st;
+    ForceArgs a = force_args(c, true);
+    a.items = L.items_q;
+    a.nitems = L.nitems;
+    a.src = L.src;
+    a.counter = counter;
+    a.acc = acc;
+    const ForceVariant v = variant(c, true);
+    const int wpg = force_waves_per_group(v.law);
+    HIPCHK(hipEventRecord(c->ev_k0, c->stream));
+    if (a.nitems > 0) {
+        HIPCHK(launch_reset_counter(counter, c->stream));
+        HIPCHK(launch_force(a, c->law.lc, v, std::min(queue_groups(c, v), (a.nitems + wpg - 1) / wpg), c->stream));
+    }
     HIPCHK(hipEventRecord(c->ev_k1, c->stream));
     c->kernel_timed = true;
     return PP2P_OK;

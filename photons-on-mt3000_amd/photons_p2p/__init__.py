@@ -62,7 +62,7 @@ CORE_SYMBOLS = (
     "pp2p_walk_tasks", "pp2p_get_tasks", "pp2p_build_tree", "pp2p_walk_tree", "pp2p_get_tree",
     "pp2p_m2l", "pp2p_l2p", "pp2p_fmm_short", "pp2p_get_expansions", "pp2p_get_m2l_tasks",
     "pp2p_pm_forces", "pp2p_pm_add_tree", "pp2p_pm_get_mesh",
-    "pp2p_prepare", "pp2p_compute", "pp2p_zero_acc", "pp2p_run_graph", "pp2p_sync",
+    "pp2p_prepare", "pp2p_compute", "pp2p_compute_noalloc", "pp2p_zero_acc", "pp2p_run_graph", "pp2p_sync",
     "pp2p_get_acc_f64", "pp2p_get_acc_f32_soa", "pp2p_list_stats", "pp2p_last_kernel_ms",
     "pp2p_last_prepare_ms", "pp2p_count_inside", "pp2p_time_steps", "pp2p_law_eval", "pp2p_get_csr",
 )
@@ -113,6 +113,7 @@ def load(path: str | None = None) -> ctypes.CDLL:
         "pp2p_pm_get_mesh": (i32, [vp, i32, vp]),
         "pp2p_prepare": (i32, [vp]),
         "pp2p_compute": (i32, [vp, i32]),
+        "pp2p_compute_noalloc": (i32, [vp, i64, vp, vp, vp]),
         "pp2p_zero_acc": (i32, [vp]),
         "pp2p_run_graph": (i32, [vp, i32, i32, i32]),
         "pp2p_sync": (i32, [vp]),
@@ -421,6 +422,13 @@ class Context:
 
     def compute(self, exec_form: int = EXEC_QUEUE):
         self._chk(self._lib.pp2p_compute(self._h, exec_form))
+
+    def compute_noalloc(self, ntask: int, task_t_dev: int, task_s_dev: int, acc_dev: int):
+        """pp2p_compute_noalloc: the list as device arrays (addresses of ntask
+        int32 each), accumulating into a device array of 3 x npart doubles;
+        the library allocates nothing for the call."""
+        self._chk(self._lib.pp2p_compute_noalloc(self._h, ntask, ctypes.c_void_p(task_t_dev),
+                                                 ctypes.c_void_p(task_s_dev), ctypes.c_void_p(acc_dev)))
 
     def zero_acc(self):
         self._chk(self._lib.pp2p_zero_acc(self._h))
