@@ -40,7 +40,7 @@ METRIC = "G pair-interactions/s + % FP32 vector peak at p99 force err <=1e-5, 1 
 # HBM bytes per force launch (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, factors from a
 # 1 GiB copy kernel profiled in the same runs); see the named profile summary
 MEASURED_TRAFFIC = {
-    "clustered256": {"bytes": 6.542e10, "profile": "profiles/p2p_force_clustered256_traffic.md"},
+    "clustered256": {"bytes": 6.557e10, "profile": "profiles/p2p_force_clustered256_traffic.md"},
 }
 
 
@@ -72,9 +72,10 @@ def sample_rows(w, nrows: int, seed: int):
     return np.isin(w.task_t, pick)
 
 
-def accuracy(w, acc, mask, threads, pos=None, act=None, level=1):
+def accuracy(w, acc, mask, threads, pos=None, act=None, level=1, softening=None):
     import pyoracle
-    prm = pyoracle.params(w.eps, w.rs, mass=w.mass, softening=w.softening, box=w.box)
+    prm = pyoracle.params(w.eps, w.rs, mass=w.mass, softening=w.softening if softening is None else softening,
+                          box=w.box)
     csr = pyoracle.build_csr(w.task_t[mask], w.task_s[mask], w.nleaf)
     ref = pyoracle.p2p_f64(prm, w.pos if pos is None else pos, w.leaf_ip, w.leaf_np, csr, act=act, level=level,
                            nthreads=threads)
@@ -210,8 +211,14 @@ def dropin_protocol(w, threads: int, len_task: int) -> dict:
         accp[d] = accs[d].ctypes.data
     hl = W._host_lib()
     P = lambda a: a.ctypes.data_as(vp)  # noqa: E731
-    posi = np.ascontiguousarray(m["posi"], np.int32)
-    tag = np.ascontiguousarray(m["tag"], np.int32)
+    # the tree is built from the particles in their input order, as the
+    # workload's was: the partition's outcome depends on that order, and the
+    # leaves must be the Pack records' (the Body records are in tree order)
+    perm = m["perm"]
+    posi = np.empty_like(np.asarray(m["posi"], np.int32))
+    posi[perm] = m["posi"]
+    tag = np.empty_like(np.asarray(m["tag"], np.int32))
+    tag[perm] = m["tag"]
     bdl = np.asarray(m["bdl"], np.float32)
     bdr = np.asarray(m["bdr"], np.float32)
     t0 = time.perf_counter()
@@ -220,7 +227,13 @@ def dropin_protocol(w, threads: int, len_task: int) -> dict:
     ip = np.empty(nl, np.int32)
     npl = np.empty(nl, np.int32)
     hl.phost_get_leaves(h, P(ip), P(npl), None, None, None)
-    same_leaves = bool(np.array_equal(ip, w.leaf_ip) and np.array_equal(npl, w.leaf_np))
+    hperm = np.empty(n, np.int32)
+    hl.phost_get_perm(h, P(hperm))
+    same_leaves = bool(np.array_equal(ip, w.leaf_ip) and np.array_equal(npl, w.leaf_np)
+                       and np.array_equal(hperm, perm))
+    if not same_leaves:
+        hl.phost_free(h)
+        raise RuntimeError("drop-in step: the host tree's leaves differ from the Pack records")
     fn = ctypes.cast(lib.task_compute_p2p_gpu, vp)
     res = {"len_task": len_task, "host_tree_s": tree_s, "leaves_identical": same_leaves, "host_threads": threads}
     try:
@@ -348,6 +361,7 @@ def main():
     explicit = args.waves_per_cu or args.split or args.chunk_model or args.segs or args.no_cull or args.ipl_max
     if args.autotune:
         res = AT.tune(ctx, lawp, log=log)
+        law = res["best"].get("law", law)
         tuned = {"source": "autotune", "signature": sig, **res}
         log(f"[bench] autotuned {sig}: {res['best']} ({res['ms']:.3f} ms/step)")
         if args.tuned_out:
@@ -709,7 +723,9 @@ def main():
         acc_d = d.pop("acc")
         if not args.no_acc:
             mask = sample_rows(w, args.acc_rows, seed=20261202)
-            (md, pd, xd), nd = accuracy(w, acc_d, mask, threads, pos=pos_eval)
+            # the drop-in applies the reference's max(r, SoftenScale) softening
+            # (src/gpu_kernel.impl.h:66-81) whatever the workload's
+            (md, pd, xd), nd = accuracy(w, acc_d, mask, threads, pos=pos_eval, softening=0)
             d["rel_err"] = {"median": md, "p99": pd, "max": xd, "particles": int(nd)}
         del acc_d
         d["step_host_lists_ms_before"] = lists["step_host_lists_ms"] if lists else None

@@ -23,6 +23,7 @@ hipError_t launch_i32_to_f64(const int *posi, long long n3, double int2pos, bool
                              hipStream_t st);
 hipError_t launch_round_f32(double *x3, long long n3, hipStream_t st);
 hipError_t launch_fill_f64(double *p, long long n, double v, hipStream_t st);
+hipError_t launch_f64_to_f32(const double *a, long long n, float *b, hipStream_t st);
 hipError_t launch_fill_i32(int *p, long long n, int v, hipStream_t st);
 
 // The functions below follow the rocPRIM convention: temp == nullptr only

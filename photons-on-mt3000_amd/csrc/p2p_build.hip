@@ -430,6 +430,20 @@ hipError_t launch_round_f32(double *x3, long long n3, hipStream_t st)
     return hipGetLastError();
 }
 
+__global__ void k_f64_to_f32(const double *__restrict__ a, long long n, float *__restrict__ b)
+{
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long long)gridDim.x * blockDim.x)
+        b[k] = (float)a[k];
+}
+
+hipError_t launch_f64_to_f32(const double *a, long long n, float *b, hipStream_t st)
+{
+    if (n <= 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_f64_to_f32, dim3(fill_grid(n)), dim3(256), 0, st, a, n, b);
+    return hipGetLastError();
+}
+
 hipError_t launch_fill_f64(double *p, long long n, double v, hipStream_t st)
 {
     if (n <= 0)

@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/photons_p2p.h"
@@ -68,6 +69,48 @@ struct DevBuf {
     }
 };
 
+// grow-only pinned host buffer (staging of host-side repacking)
+struct PinnedBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    hipError_t reserve(size_t bytes)
+    {
+        if (bytes <= cap)
+            return hipSuccess;
+        release();
+        hipError_t e = hipHostMalloc(&p, std::max<size_t>(bytes, 1), hipHostMallocDefault);
+        if (e == hipSuccess)
+            cap = bytes;
+        else
+            p = nullptr;
+        return e;
+    }
+    void release()
+    {
+        if (p)
+            (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+// fn(b, e) over [0, n) in up to 16 threads when n is large
+template <typename Fn>
+void parallel_ranges(long long n, Fn fn)
+{
+    const unsigned hw = std::thread::hardware_concurrency();
+    const long long nth = n < (1ll << 20) ? 1 : std::min<long long>(hw ? hw : 1, 16);
+    if (nth <= 1) {
+        fn(0ll, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (long long k = 0; k < nth; ++k)
+        th.emplace_back(fn, n * k / nth, n * (k + 1) / nth);
+    for (auto &t : th)
+        t.join();
+}
+
 }  // namespace
 
 struct pp2p_ctx {
@@ -93,8 +136,10 @@ struct pp2p_ctx {
     DevBuf<double4> ref;
     DevBuf<int2> rng;
     DevBuf<double> acc;
+    DevBuf<float> acc32;        // FP32 copy of acc for pp2p_get_acc_f32_soa
     DevBuf<double> x3;          // FP64 staging of positions
     DevBuf<int> posi;           // int32 staging
+    PinnedBuf h_stage;          // host packing of strided records
     std::vector<int2> h_rng;
 
     // active subset
@@ -295,6 +340,8 @@ void pp2p_destroy(pp2p_ctx *c)
     c->nsx.release();
     c->idx.release(); c->idx_sorted.release(); c->counter.release(); c->temp.release();
     c->pair_count.release();
+    c->acc32.release();
+    c->h_stage.release();
     if (c->ev_k0) (void)hipEventDestroy(c->ev_k0);
     if (c->ev_k1) (void)hipEventDestroy(c->ev_k1);
     if (c->ev_p0) (void)hipEventDestroy(c->ev_p0);
@@ -476,20 +523,23 @@ int pp2p_set_particles_i32(pp2p_ctx *c, long long npart, const void *posi, size_
     if (npart) {
         // one contiguous copy: records wider than the three coordinates are
         // packed on the host first (as the reference repacks part[i].posi,
-        // src/gpu_entry.cu:85-105); a pitched copy of npart 12-byte rows from
-        // pageable memory is orders of magnitude slower
+        // src/gpu_entry.cu:85-105), in parallel, into a pinned buffer kept by
+        // the context; a pitched copy of npart 12-byte rows from pageable
+        // memory is orders of magnitude slower
         const void *src = posi;
-        std::vector<int> packed;
         if (stride != 12) {
-            packed.resize(3 * (size_t)npart);
+            HIPCHK(c->h_stage.reserve(12 * (size_t)npart));
+            int *packed = static_cast<int *>(c->h_stage.p);
             const unsigned char *b = static_cast<const unsigned char *>(posi);
-            for (long long i = 0; i < npart; ++i)
-                std::memcpy(&packed[3 * (size_t)i], b + (size_t)i * stride, 12);
-            src = packed.data();
+            parallel_ranges(npart, [&](long long i0, long long i1) {
+                for (long long i = i0; i < i1; ++i)
+                    std::memcpy(&packed[3 * (size_t)i], b + (size_t)i * stride, 12);
+            });
+            src = packed;
         }
         HIPCHK(hipMemcpyAsync(c->posi.p, src, 12 * (size_t)npart, hipMemcpyHostToDevice, c->stream));
         HIPCHK(launch_i32_to_f64(c->posi.p, 3 * npart, int2pos, false, c->x3.p, c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));      // before the packed copy goes out of scope
+        HIPCHK(hipStreamSynchronize(c->stream));      // before the staging buffer is reused
     }
     return finish_particles(c, npart, nleaf);
 }
@@ -535,9 +585,37 @@ static int append_tasks(pp2p_ctx *c, long long n, const int *tt, const int *ts)
         return fail(PP2P_ERR_ARG, "invalid task arrays");
     if (c->ntask + n > (1ll << 31) - 1)
         return fail(PP2P_ERR_ARG, "more than 2^31-1 tasks in one list");
-    for (long long k = 0; k < n; ++k)
-        if (tt[k] < 0 || tt[k] >= c->nleaf || ts[k] < 0 || ts[k] >= c->nleaf)
-            return fail(PP2P_ERR_ARG, "task " + std::to_string(k) + " names a leaf outside [0, nleaf)");
+    {
+        // the first task naming a leaf outside [0, nleaf), in parallel over
+        // ranges of a long list
+        const unsigned nl = (unsigned)c->nleaf;
+        auto first_bad = [&](long long b, long long e) {
+            for (long long k = b; k < e; ++k)
+                if ((unsigned)tt[k] >= nl || (unsigned)ts[k] >= nl)
+                    return k;
+            return e;
+        };
+        long long bad = n;
+        const unsigned hw = std::thread::hardware_concurrency();
+        const int nth = n < (1ll << 20) ? 1 : (int)std::min<unsigned>(hw ? hw : 1, 16);
+        if (nth <= 1) {
+            bad = first_bad(0, n);
+        } else {
+            std::vector<long long> found(nth, n);
+            std::vector<std::thread> th;
+            for (int k = 0; k < nth; ++k)
+                th.emplace_back([&, k] {
+                    const long long b = n * k / nth, e = n * (k + 1) / nth;
+                    const long long f = first_bad(b, e);
+                    found[k] = f < e ? f : n;
+                });
+            for (auto &t : th)
+                t.join();
+            bad = *std::min_element(found.begin(), found.end());
+        }
+        if (bad < n)
+            return fail(PP2P_ERR_ARG, "task " + std::to_string(bad) + " names a leaf outside [0, nleaf)");
+    }
     const size_t need = (size_t)(c->ntask + n);
     if (need > c->task_t.cap) {
         // keep the existing tasks across the grow
@@ -1724,13 +1802,13 @@ int pp2p_get_acc_f32_soa(pp2p_ctx *c, float *out)
         return st;
     if (!out && c->npart)
         return fail(PP2P_ERR_ARG, "null output");
-    std::vector<double> planes((size_t)3 * c->npart);
-    if (c->npart)
-        HIPCHK(hipMemcpyAsync(planes.data(), c->acc.p, sizeof(double) * 3 * c->npart, hipMemcpyDeviceToHost,
-                              c->stream));
+    // rounded to FP32 on the device: half the bytes over the bus, no host pass
+    if (c->npart) {
+        HIPCHK(c->acc32.reserve((size_t)3 * c->npart));
+        HIPCHK(launch_f64_to_f32(c->acc.p, 3 * c->npart, c->acc32.p, c->stream));
+        HIPCHK(hipMemcpyAsync(out, c->acc32.p, sizeof(float) * 3 * c->npart, hipMemcpyDeviceToHost, c->stream));
+    }
     HIPCHK(hipStreamSynchronize(c->stream));
-    for (size_t k = 0; k < planes.size(); ++k)
-        out[k] = (float)planes[k];
     return PP2P_OK;
 }
 

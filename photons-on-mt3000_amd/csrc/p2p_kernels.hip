@@ -279,6 +279,19 @@ __device__ __forceinline__ void process_item(const ForceArgs &a, const LawConst 
             }
         }
     };
+    // FP32 partial sums fold into FP64 once at least 64 steps have gone into
+    // them (a step adds one pair per slot), so every FP32 sum holds at most
+    // 64 + 63 terms whatever the batch length; a batch of a many-phase item
+    // is a few steps only.
+    int unfolded = 0;                               // steps in the FP32 sums (wave-uniform)
+    auto fold = [&]() {
+#pragma unroll
+        for (int k = 0; k < IPL; ++k) {
+            sx[k] += (double)fx[k]; sy[k] += (double)fy[k]; sz[k] += (double)fz[k];
+            fx[k] = 0.f; fy[k] = 0.f; fz[k] = 0.f;
+        }
+        unfolded = 0;
+    };
     // one batch of n (a multiple of G) tile entries, the pointer stepping by
     // G.  One loop serves every phase count: specialised copies per count
     // measured slower, the kernel outgrowing the instruction cache.
@@ -290,13 +303,9 @@ __device__ __forceinline__ void process_item(const ForceArgs &a, const LawConst 
             pairs_of(*tp);
             tp += G;
         }
-        if (FOLD64) {
-#pragma unroll
-            for (int k = 0; k < IPL; ++k) {
-                sx[k] += (double)fx[k]; sy[k] += (double)fy[k]; sz[k] += (double)fz[k];
-                fx[k] = 0.f; fy[k] = 0.f; fz[k] = 0.f;
-            }
-        }
+        unfolded += ch;
+        if (FOLD64 && unfolded >= kWave)
+            fold();
     };
 
     // Sources 64 at a time: lane s holds source s's first particle (relative
@@ -418,6 +427,8 @@ __device__ __forceinline__ void process_item(const ForceArgs &a, const LawConst 
         n_in_total += n_in;
         return;
     }
+    if (FOLD64 && unfolded > 0)
+        fold();
 #pragma unroll
     for (int k = 0; k < IPL; ++k) {
         if (!FOLD64) {

@@ -13,11 +13,14 @@
 // go to the device as one list.  Results stay on the device until
 // devMemcpyDH_updateAcc adds them into acc[]; every entry point that reads
 // results or device state first waits for the queued work.
+#include <hip/hip_runtime_api.h>
+
 #include <condition_variable>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <deque>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -43,6 +46,67 @@ namespace {
 
 const double kUcut = 2.8125;   // GPUCUTOFF, src/gpu_kernel.impl.h:86
 
+// Host array in pinned memory (hipHostMalloc), so that copies to and from
+// the device run at the link's rate; pageable memory when pinning fails.
+// Uninitialised: every entry used is written first.
+template <typename T>
+class HostArray {
+public:
+    HostArray() = default;
+    HostArray(const HostArray &) = delete;
+    HostArray &operator=(const HostArray &) = delete;
+    HostArray(HostArray &&o) noexcept { swap(o); }
+    HostArray &operator=(HostArray &&o) noexcept
+    {
+        if (this != &o) {
+            release();
+            swap(o);
+        }
+        return *this;
+    }
+    ~HostArray() { release(); }
+    T *get() const { return p_; }
+    size_t size() const { return cap_; }
+    void fit(size_t n)
+    {
+        if (n <= cap_)
+            return;
+        release();
+        void *q = nullptr;
+        if (hipHostMalloc(&q, sizeof(T) * n, hipHostMallocDefault) == hipSuccess && q) {
+            p_ = static_cast<T *>(q);
+            pinned_ = true;
+        } else {
+            (void)hipGetLastError();
+            p_ = new T[n];
+            pinned_ = false;
+        }
+        cap_ = n;
+    }
+    void release()
+    {
+        if (p_) {
+            if (pinned_)
+                (void)hipHostFree(p_);
+            else
+                delete[] p_;
+        }
+        p_ = nullptr;
+        cap_ = 0;
+    }
+
+private:
+    void swap(HostArray &o)
+    {
+        std::swap(p_, o.p_);
+        std::swap(cap_, o.cap_);
+        std::swap(pinned_, o.pinned_);
+    }
+    T *p_ = nullptr;
+    size_t cap_ = 0;
+    bool pinned_ = false;
+};
+
 struct Step {
     pp2p_ctx *ctx = nullptr;
     int device = -1;
@@ -53,6 +117,7 @@ struct Step {
     bool failed = false;           // the step's set-up failed: its flushes are dropped (worker)
     std::vector<float> out;        // hostmalloc_out: 3 planes of the copied results
     long long out_n = 0;
+    HostArray<float> soa;          // the rank's results, 3 planes of NPART (kept across steps)
     std::vector<int> lip, lnp;
     // p2ppack_compute_*: per pack results, by pack index
     std::vector<std::vector<double>> pack_acc;
@@ -149,12 +214,50 @@ bool begin_step(bool with_act, int active_level)
     return check(pp2p_set_tasks(g.ctx, 0, nullptr, nullptr), "set_tasks");
 }
 
+// Task storage of one flush, kept for reuse: a flush of LEN_TASK tasks is
+// tens of MB, and fresh memory would be pinned (or page-faulted) every time.
+struct TaskBuf {
+    HostArray<int> t, s;
+    size_t n = 0;
+    void fit(size_t want)
+    {
+        t.fit(want);
+        s.fit(want);
+        n = want;
+    }
+};
+
+class BufPool {
+public:
+    TaskBuf take()
+    {
+        std::lock_guard<std::mutex> lk(m_);
+        if (free_.empty())
+            return TaskBuf{};
+        TaskBuf b = std::move(free_.back());
+        free_.pop_back();
+        return b;
+    }
+    void give(TaskBuf &&b)
+    {
+        std::lock_guard<std::mutex> lk(m_);
+        if (free_.size() < 8)
+            free_.push_back(std::move(b));
+    }
+
+private:
+    std::mutex m_;
+    std::vector<TaskBuf> free_;
+};
+
+BufPool g_pool;
+
 // One unit of device work, handed from the caller's thread to the worker.
 struct Job {
     enum Kind { BEGIN, FLUSH } kind = FLUSH;
     bool act = false;              // BEGIN: active step
     int level = 0;                 // BEGIN: active level
-    std::vector<int> tt, ts;       // FLUSH: leaf pairs, ids relative to first_leaf
+    TaskBuf tasks;                 // FLUSH: leaf pairs, ids relative to first_leaf
 };
 
 // Runs on the worker.  BEGIN: the step's set-up.  A batch of FLUSH jobs: one
@@ -169,11 +272,15 @@ void run_jobs(std::vector<Job> &batch)
     }
     if (g.failed || !g.ctx)
         return;
-    bool ok = check(pp2p_set_tasks(g.ctx, (long long)batch[0].tt.size(), batch[0].tt.data(), batch[0].ts.data()),
+    bool ok = check(pp2p_set_tasks(g.ctx, (long long)batch[0].tasks.n, batch[0].tasks.t.get(),
+                                   batch[0].tasks.s.get()),
                     "set_tasks");
     for (size_t k = 1; ok && k < batch.size(); ++k)
-        ok = check(pp2p_add_tasks(g.ctx, (long long)batch[k].tt.size(), batch[k].tt.data(), batch[k].ts.data()),
+        ok = check(pp2p_add_tasks(g.ctx, (long long)batch[k].tasks.n, batch[k].tasks.t.get(), batch[k].tasks.s.get()),
                    "add_tasks");
+    // the tasks are on the device now: their host storage goes back to the pool
+    for (Job &j : batch)
+        g_pool.give(std::move(j.tasks));
     if (ok)
         ok = check(pp2p_compute(g.ctx, PP2P_EXEC_QUEUE), "compute");
     if (ok) {
@@ -290,11 +397,27 @@ void *flush_call(int ntask, int *lt, int *ls, int last_time, bool act, int activ
             // buffers as soon as the call returns, and they are not rebased
             Job f;
             f.kind = Job::FLUSH;
-            f.tt.resize(ntask);
-            f.ts.resize(ntask);
-            for (int k = 0; k < ntask; ++k) {
-                f.tt[k] = lt[k] - first_leaf;
-                f.ts[k] = ls[k] - first_leaf;
+            f.tasks = g_pool.take();
+            f.tasks.fit((size_t)ntask);
+            int *ft = f.tasks.t.get(), *fs = f.tasks.s.get();
+            const int base = first_leaf;
+            auto rebase = [=](long long b, long long e) {
+                for (long long k = b; k < e; ++k) {
+                    ft[k] = lt[k] - base;
+                    fs[k] = ls[k] - base;
+                }
+            };
+            // the walk waits for this copy: spread a long flush over threads
+            const unsigned hw = std::thread::hardware_concurrency();
+            const long long nth = ntask < (1 << 20) ? 1 : std::min<long long>(hw ? hw : 1, 8);
+            if (nth <= 1) {
+                rebase(0, ntask);
+            } else {
+                std::vector<std::thread> th;
+                for (long long k = 0; k < nth; ++k)
+                    th.emplace_back(rebase, ntask * k / nth, ntask * (k + 1) / nth);
+                for (auto &t : th)
+                    t.join();
             }
             worker.post(std::move(f));
             g.tot_tasks += ntask;
@@ -368,6 +491,33 @@ void *pack_call(P2PPack *packs, int npack, bool act, int active_level)
 bool level_selected(int j, int active_level)
 {
     return active_level == -1 || (active_level && part[j].act == active_level);
+}
+
+// acc[d][ip + i] += x[d][i] for the selected particles of [ip, ip + n), in
+// parallel over disjoint ranges when n is large
+void add_results(long long ip, long long n, const float *x, const float *y, const float *z, int active_level)
+{
+    auto run = [&](long long b, long long e) {
+        for (long long i = b; i < e; ++i) {
+            const int j = (int)(ip + i);
+            if (level_selected(j, active_level)) {
+                acc[0][j] += x[i];
+                acc[1][j] += y[i];
+                acc[2][j] += z[i];
+            }
+        }
+    };
+    const unsigned hw = std::thread::hardware_concurrency();
+    const long long nth = n < (1 << 20) ? 1 : std::min<long long>(hw ? hw : 1, 16);
+    if (nth <= 1) {
+        run(0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (long long k = 0; k < nth; ++k)
+        th.emplace_back(run, n * k / nth, n * (k + 1) / nth);
+    for (auto &t : th)
+        t.join();
 }
 
 }  // namespace
@@ -462,6 +612,7 @@ void devfree(void)
     g.ctx = nullptr;
     g.call_cnt = 0;
     g.pack_acc.clear();
+    g.soa.release();
 }
 
 void updateAcc_fromHostBuff(P2PPack pack, int active_level)
@@ -471,14 +622,7 @@ void updateAcc_fromHostBuff(P2PPack pack, int active_level)
         set_error(PP2P_ERR_STATE, "updateAcc_fromHostBuff: hostmalloc_out buffer too small");
         return;
     }
-    for (long long i = 0; i < n; ++i) {
-        const int j = pack.ipart + (int)i;
-        if (level_selected(j, active_level)) {
-            acc[0][j] += g.out[i];
-            acc[1][j] += g.out[i + n];
-            acc[2][j] += g.out[i + 2 * n];
-        }
-    }
+    add_results(pack.ipart, n, g.out.data(), g.out.data() + n, g.out.data() + 2 * n, active_level);
 }
 
 void devMemcpyDH_updateAcc(P2PPack *packs, int npack, int active_level)
@@ -500,9 +644,11 @@ void devMemcpyDH_updateAcc(P2PPack *packs, int npack, int active_level)
         }
         return;
     }
-    std::vector<float> soa((size_t)3 * NPART);
-    if (!check(pp2p_get_acc_f32_soa(g.ctx, soa.data()), "get_acc"))
+    // the device rounds to FP32; the pinned host buffer is kept across steps
+    g.soa.fit((size_t)3 * (NPART > 0 ? NPART : 1));
+    if (!check(pp2p_get_acc_f32_soa(g.ctx, g.soa.get()), "get_acc"))
         return;
+    const float *soa = g.soa.get();
     for (int n = 0; n < npack; ++n) {
         if (packs[n].ntask == 0)
             continue;
@@ -511,11 +657,7 @@ void devMemcpyDH_updateAcc(P2PPack *packs, int npack, int active_level)
             set_error(PP2P_ERR_ARG, "pack particle range outside [0, NPART)");
             return;
         }
-        if ((long long)g.out.size() < 3 * np)
-            hostmalloc_out((int)np);
-        for (int d = 0; d < 3; ++d)
-            std::memcpy(&g.out[(size_t)d * np], &soa[(size_t)d * NPART + ip], sizeof(float) * np);
-        updateAcc_fromHostBuff(packs[n], active_level);
+        add_results(ip, np, soa + ip, soa + (size_t)NPART + ip, soa + 2 * (size_t)NPART + ip, active_level);
     }
 }
 

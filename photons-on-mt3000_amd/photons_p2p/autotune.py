@@ -8,7 +8,8 @@ fastest.  The knobs are the ones pp2p_tuning exposes:
                owns before it fetches the next item
   waves_per_cu persistent waves per CU of the queue form (0 = occupancy max)
   law          the cubic LOGTAB table, the linear table (LINEAR) or the
-               table-free POLY law
+               table-free POLY law; a law other than the cubic is kept only
+               when its forces stay within LAW_GAP_P99 of the cubic's
 Tuned values are kept in photons_p2p/tuned.json, keyed by a signature of
 the workload shape (particle, leaf, task and pair counts in half-octave
 buckets); `lookup` returns the entry for a shape, `tune` measures a new one.
@@ -34,6 +35,32 @@ GRID = {
 def signature(ntask: int, npart: int, nleaf: int, npairs: int) -> str:
     b = lambda x: int(round(2 * __import__("math").log2(max(x, 1))))  # noqa: E731
     return f"p{b(npart)}-l{b(nleaf)}-t{b(ntask)}-n{b(npairs)}"
+
+
+# A law other than the cubic table is kept only if its accelerations differ
+# from the cubic's by at most this much (p99 over the list's target
+# particles, relative): half the 1e-5 force-error budget, so that the cubic's
+# own error (1-3e-6) plus the difference stays inside it.
+LAW_GAP_P99 = 5e-6
+
+
+def law_gap(ctx: Context, law: ForceLaw, cfg: dict) -> float:
+    """p99 of |a_cfg - a_cubic| / |a_cubic| over the particles that receive a
+    force, both evaluated on the device for the context's list."""
+    import numpy as np
+    apply(ctx, law, {**cfg, "law": LAW_TABLE})
+    ctx.zero_acc()
+    ctx.compute(EXEC_QUEUE)
+    ref = ctx.acc()
+    apply(ctx, law, cfg)
+    ctx.zero_acc()
+    ctx.compute(EXEC_QUEUE)
+    a = ctx.acc()
+    mag = np.linalg.norm(ref, axis=1)
+    sel = mag > 0
+    if not np.any(sel):
+        return 0.0
+    return float(np.percentile(np.linalg.norm(a[sel] - ref[sel], axis=1) / mag[sel], 99))
 
 
 def _time(ctx: Context, steps: int) -> float:
@@ -70,9 +97,17 @@ def tune(ctx: Context, law: ForceLaw, steps: int = 3, grid: dict | None = None, 
                 set_cfg(best)
                 continue
             ms = _time(ctx, steps)
-            trials.append({**cfg, "ms": ms})
+            trial = {**cfg, "ms": ms}
+            if ms < best_ms and knob == "law" and v != LAW_TABLE:
+                gap = law_gap(ctx, law, cfg)
+                trial["law_gap_p99"] = gap
+                if gap > LAW_GAP_P99:
+                    trial["rejected"] = "accuracy"
+                    ms = float("inf")
+                set_cfg(cfg)
+            trials.append(trial)
             if log:
-                log(f"[autotune] {cfg} -> {ms:.3f} ms")
+                log(f"[autotune] {trial}")
             if ms < best_ms:
                 best, best_ms = cfg, ms
     set_cfg(best)

@@ -267,8 +267,12 @@ def test_walk_streams_flushes_into_the_dropin(libs):
     m = w.meta
     hl = W._host_lib()
     P = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
-    posi = np.ascontiguousarray(m["posi"], np.int32)
-    tag = np.ascontiguousarray(m["tag"], np.int32)
+    # the host program builds its tree from the particles in input order (the
+    # partition's outcome depends on the order), as the workload's was built
+    posi = np.empty_like(np.asarray(m["posi"], np.int32))
+    posi[m["perm"]] = m["posi"]
+    tag = np.empty_like(np.asarray(m["tag"], np.int32))
+    tag[m["perm"]] = m["tag"]
     h = hl.phost_build_tree(posi.shape[0], P(posi), P(tag), m["int2pos"], m["maxleaf"],
                             P(np.asarray(m["bdl"], np.float32)), P(np.asarray(m["bdr"], np.float32)), 0)
     assert h
@@ -278,6 +282,9 @@ def test_walk_streams_flushes_into_the_dropin(libs):
         assert hl.phost_count(h, 1) == w.nleaf
         hl.phost_get_leaves(h, P(ip), P(npl), None, None, None)
         assert np.array_equal(ip, w.leaf_ip) and np.array_equal(npl, w.leaf_np)
+        hperm = np.empty(w.npart, np.int32)
+        hl.phost_get_perm(h, P(hperm))
+        assert np.array_equal(hperm, m["perm"])
         len_task = 3000
         lib.hostmalloc_task(len_task)
         tt, ts = H.tasks()

@@ -187,8 +187,13 @@ def test_autotune_keeps_results_and_picks_a_measured_config(gpu_ctx):
     gpu_ctx.set_particles(w.pos, w.leaf_ip, w.leaf_np)
     gpu_ctx.set_tasks(w.task_t, w.task_s)
     res = AT.tune(gpu_ctx, law, steps=1, grid={"ipl_max": [1, 2], "split_pairs": [1 << 14], "waves_per_cu": [8],
-                                               "law": [pp.LAW_POLY]})
-    assert res["ms"] == min(t["ms"] for t in res["trials"])
+                                               "law": [pp.LAW_LINEAR, pp.LAW_POLY]})
+    kept = [t for t in res["trials"] if "rejected" not in t]
+    assert res["ms"] == min(t["ms"] for t in kept)
+    for t in res["trials"]:
+        # a faster law other than the cubic carries its measured gap to the cubic
+        if t["law"] != pp.LAW_TABLE and "law_gap_p99" in t:
+            assert ("rejected" in t) == (t["law_gap_p99"] > AT.LAW_GAP_P99)
     gpu_ctx.zero_acc()
     gpu_ctx.compute()
     med, p99, mx = pyoracle.rel_err(gpu_ctx.acc(), ref)
