@@ -307,9 +307,17 @@ __device__ __forceinline__ void chunk_of(int ni, int k, int q, int &beg, int &si
     size = base + (q < rem ? 1 : 0);
 }
 
-// Source ranges of a row: greedy cut when the running pairs of the largest
-// i-chunk would exceed `split`; split <= 0: one range.  Every (range, chunk)
-// pair is one item; its cost is the modelled lane-work IPL * 64 / G * sum n_s.
+// A staged source particle costs a wave about as much as this many pair
+// evaluations: its frame shift, cull and compaction and the window walk take
+// about 4 SIMD cycles, a pair about 0.4.
+constexpr long long kStagePairs = 8;
+
+// Source ranges of a row: greedy cut when the running work of the largest
+// i-chunk, (n_i + kStagePairs) per source particle, would exceed `split`;
+// split <= 0: one range.  Counting the staging keeps a small target with a
+// long source list from becoming one long item (it would run last in the
+// cost order and hold the launch open).  Every (range, chunk) pair is one
+// item; its cost is the modelled work (IPL * 64 / G + kStagePairs) * sum n_s.
 template <bool EMIT>
 __device__ int row_ranges(const RowView &v, int r, long long split, int tgt, int ibeg, int ni,
                           Item *items, unsigned *cost, int out0)
@@ -328,14 +336,15 @@ __device__ int row_ranges(const RowView &v, int r, long long split, int tgt, int
             item_shape(cs, v.ipl_max, lanes, ipl);
             items[out] = Item{tgt, ibeg + cb, cs, r, s_beg, s_end, lanes, ipl};
             if (cost)
-                cost[out] = (unsigned)min(ns * (long long)(kWave * ipl / (kWave / lanes)), 0xffffffffll);
+                cost[out] = (unsigned)min(ns * ((long long)(kWave * ipl / (kWave / lanes)) + kStagePairs),
+                                          0xffffffffll);
             ++out;
         }
     };
     // the cut after `beg`: a binary search on the prefix of the source sizes
     const unsigned *P = v.nsx;
     for (;;) {
-        const int cut = split > 0 ? range_cut(PrefixArray{P}, beg, k1, wi, split) : k1;
+        const int cut = split > 0 ? range_cut(PrefixArray{P}, beg, k1, wi + kStagePairs, split) : k1;
         ++nr;
         if (cut >= k1) {
             if (EMIT)
