@@ -64,9 +64,14 @@ def law_gap(ctx: Context, law: ForceLaw, cfg: dict) -> float:
 
 
 def _time(ctx: Context, steps: int) -> float:
+    """ms per step: at least `steps` steps and at least about 50 ms of device
+    time, so that short steps (a fraction of a millisecond) are not timed
+    from a handful of launches."""
     ctx.time_steps(EXEC_QUEUE, 1)                       # warm (builds the queue for this tuning)
-    tot, _ = ctx.time_steps(EXEC_QUEUE, steps)
-    return tot / steps
+    est, _ = ctx.time_steps(EXEC_QUEUE, 1)
+    n = max(steps, min(400, int(50.0 / max(est, 1e-3))))
+    tot, _ = ctx.time_steps(EXEC_QUEUE, n)
+    return tot / n
 
 
 def tune(ctx: Context, law: ForceLaw, steps: int = 3, grid: dict | None = None, log=None,
