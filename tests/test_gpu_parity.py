@@ -89,12 +89,13 @@ def test_segments_16_and_no_cull(gpu_ctx):
     gpu_ctx.set_tuning()
 
 
-def test_active_subset(gpu_ctx):
+@pytest.mark.parametrize("law", [pp.LAW_TABLE, pp.LAW_LINEAR])
+def test_active_subset(gpu_ctx, law):
     w = _small_uneven(seed=14)
     g = np.random.default_rng(5)
     act = (g.random(w.npart) < 0.1).astype(np.uint8) * 3
     ref = _oracle(w, act=act, level=3)
-    a = _run(gpu_ctx, w, act=act, level=3)
+    a = _run(gpu_ctx, w, law=law, act=act, level=3)
     sel = act == 3
     assert np.all(a[~sel] == 0.0)
     med, p99, mx = pyoracle.rel_err(a[sel], ref[sel])
@@ -102,9 +103,10 @@ def test_active_subset(gpu_ctx):
     assert p99 <= P99_TOL and mx <= MAX_TOL
 
 
-def test_graph_replay_matches_eager(gpu_ctx):
+@pytest.mark.parametrize("law", [pp.LAW_TABLE, pp.LAW_LINEAR])
+def test_graph_replay_matches_eager(gpu_ctx, law):
     w = _small_uneven(seed=15)
-    eager = _run(gpu_ctx, w)
+    eager = _run(gpu_ctx, w, law=law)
     for _ in range(3):
         gpu_ctx.run_graph(pp.EXEC_QUEUE, nsteps=3, nreplay=2)
         rep = gpu_ctx.acc()
@@ -114,7 +116,8 @@ def test_graph_replay_matches_eager(gpu_ctx):
         gpu_ctx.sync()
 
 
-def test_abs32_frame_matches_reference_form(gpu_ctx):
+@pytest.mark.parametrize("law", [pp.LAW_TABLE, pp.LAW_LINEAR])
+def test_abs32_frame_matches_reference_form(gpu_ctx, law):
     """FP32 absolute coordinates (the reference's scheme, open boundaries):
     the device in that frame, and the FP32 restatement of the reference
     kernel (oracle_p2p_ref32: P2P_kernel / P2P_core_1, 512-node table and
@@ -124,7 +127,7 @@ def test_abs32_frame_matches_reference_form(gpu_ctx):
     and <= 1e-4) and agree with the restatement to p99 <= 2e-4."""
     w = _small_uneven(seed=16)
     w.pos = w.pos.astype(np.float32).astype(np.float64)
-    a = _run(gpu_ctx, w, frame=pp.FRAME_ABS32, box=0.0)
+    a = _run(gpu_ctx, w, law=law, frame=pp.FRAME_ABS32, box=0.0)
     ref64 = _oracle(w, box=0.0)
     prm = pyoracle.params(w.eps, w.rs, mass=w.mass, softening=w.softening)
     csr = pyoracle.build_csr(w.task_t, w.task_s, w.nleaf)
