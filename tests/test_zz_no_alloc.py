@@ -21,7 +21,10 @@ def _hip_lib():
     global _hip
     if _hip is None:
         pp.load()                                   # brings in the runtime the library links
-        _hip = ctypes.CDLL("libamdhip64.so")
+        # that same runtime (not another copy that a name lookup might find)
+        with open("/proc/self/maps") as f:
+            mapped = [ln.split()[-1] for ln in f if "libamdhip64" in ln and "/" in ln]
+        _hip = ctypes.CDLL(mapped[0] if mapped else "libamdhip64.so")
         _hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
         _hip.hipFree.argtypes = [ctypes.c_void_p]
         _hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
