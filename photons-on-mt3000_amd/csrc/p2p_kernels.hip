@@ -8,7 +8,11 @@
 // the target leaf's frame with an FP64 offset (minimum image when the box is
 // periodic), culled against the item's bounding box and compacted.  Targets
 // smaller than 64 lane slots split every batch of the tile over G lane groups
-// ("phases") that meet by a butterfly at the end.  Pair arithmetic is FP32:
+// ("phases") that meet by a butterfly at the end.  When the phases are whole
+// quads a lane reads one tile entry per four steps and the steps share them
+// by DPP quad broadcasts folded into the first subtraction (the tile reads
+// were 40% of a step's LDS bytes on the linear law at three i-particles per
+// lane).  Pair arithmetic is FP32:
 // the LOGTAB law needs 13 FP32 and 3 integer VALU operations per pair and no
 // transcendental; FP32 partial sums fold into FP64 once per batch.
 //
@@ -133,13 +137,9 @@ __device__ __forceinline__ float2 linear_segment(float s, const char *__restrict
 // One pair, POLY law: F = exp(-u^2) h(r) / r^3 with h(r) = erfcx(u) + (2/sqrt(pi)) u
 // as a degree-12 polynomial in r = sqrt(s) in [0, 1]; explicit cutoff select.
 template <int SOFT, bool SCALE>
-__device__ __forceinline__ void pair_poly(float xi, float yi, float zi, float4 pj,
-                                          const LawConst &lc,
+__device__ __forceinline__ void pair_poly(float dx, float dy, float dz, const LawConst &lc,
                                           float &ax, float &ay, float &az)
 {
-    const float dx = pj.x - xi;
-    const float dy = pj.y - yi;
-    const float dz = pj.z - zi;
     const float s = soft_s<SOFT, SCALE>(dx, dy, dz, lc);
     const float rinv = __builtin_amdgcn_rsqf(s);
     const float r = s * rinv;
@@ -158,6 +158,24 @@ __device__ __forceinline__ void pair_poly(float xi, float yi, float zi, float4 p
 // A j-particle far beyond the cutoff of any i: pads a batch to a multiple of
 // the number of phases and contributes exactly zero.
 __device__ __forceinline__ float4 null_particle() { return make_float4(1e18f, 1e18f, 1e18f, 0.f); }
+
+// Lane U of every quad to the four lanes of the quad (DPP quad_perm
+// [U,U,U,U]).  Used as the first operand of a subtraction that is its only
+// use, the move folds into it (v_sub_f32_dpp) and costs no VALU operation.
+template <int U>
+__device__ __forceinline__ float quad_bcast(float v)
+{
+    constexpr int ctrl = U | (U << 2) | (U << 4) | (U << 6);
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), ctrl, 0xf, 0xf, true));
+}
+
+// Step U of a block of four: the j-particle held by lane U of the quad,
+// as its differences to i-slot 0
+template <int U, typename F>
+__device__ __forceinline__ void quad_step(const float4 &mine, float x0, float y0, float z0, F &pairs)
+{
+    pairs(quad_bcast<U>(mine.x) - x0, quad_bcast<U>(mine.y) - y0, quad_bcast<U>(mine.z) - z0);
+}
 
 #ifndef PP2P_WAVES_PER_EU
 #define PP2P_WAVES_PER_EU 4
@@ -184,7 +202,11 @@ __device__ __forceinline__ void process_item(const ForceArgs &a, const LawConst 
     const int q = lane - phase * L;
     const bool live = phase < G;
     const int rphase = live ? phase : 0;            // idle lanes shadow phase 0 with dummy i-slots
-    const int nb = G * (kWave / G);                 // batch: the largest multiple of G <= 64
+    // quad broadcast (below) when every quad lies in one phase
+    const bool quads = (L & 3) == 0;
+    // batch: the largest multiple of G (of 4 G with quads) <= 64
+    const int bq = quads ? 4 * G : G;
+    const int nb = bq * (kWave / bq);
     const unsigned long long below = (1ull << lane) - 1ull;
 
     float xi[IPL], yi[IPL], zi[IPL];
@@ -231,15 +253,31 @@ __device__ __forceinline__ void process_item(const ForceArgs &a, const LawConst 
 
     const unsigned seg_mask = 0xffffffffu << lc.seg_shift;
     const int tab_shift = lc.tab_shift;
-    // the IPL pairs of one j: all table addresses first, then the gathers,
-    // then the polynomials, so that the gathers of a step are in flight together
-    auto pairs_of = [&](const float4 &pj) {
+    // Slot k's differences come from slot 0's: d_k = d_0 + (x_0 - x_k), so a
+    // step computes one difference per axis from the j-particle and the DPP
+    // broadcast that supplies it has one use.  The extra rounding is that of
+    // FP32 coordinates inside one leaf, the precision the positions carry.
+    float oxi[IPL], oyi[IPL], ozi[IPL];
+#pragma unroll
+    for (int k = 0; k < IPL; ++k) {
+        oxi[k] = xi[0] - xi[k]; oyi[k] = yi[0] - yi[k]; ozi[k] = zi[0] - zi[k];
+    }
+    // the IPL pairs of one j given its differences to slot 0: all table
+    // addresses first, then the gathers, then the polynomials, so that the
+    // gathers of a step are in flight together
+    auto pairs_of = [&](float dx0, float dy0, float dz0) {
+        float dx[IPL], dy[IPL], dz[IPL];
+#pragma unroll
+        for (int k = 0; k < IPL; ++k) {
+            dx[k] = k == 0 ? dx0 : dx0 + oxi[k];
+            dy[k] = k == 0 ? dy0 : dy0 + oyi[k];
+            dz[k] = k == 0 ? dz0 : dz0 + ozi[k];
+        }
         if (!COUNT && LAW == LAW_LINEAR) {
-            float dx[IPL], dy[IPL], dz[IPL], sv[IPL];
+            float sv[IPL];
             float2 c[IPL];
 #pragma unroll
             for (int k = 0; k < IPL; ++k) {
-                dx[k] = pj.x - xi[k]; dy[k] = pj.y - yi[k]; dz[k] = pj.z - zi[k];
                 sv[k] = soft_s<SOFT, !FOLD64>(dx[k], dy[k], dz[k], lc);
                 c[k] = linear_segment(sv[k], lds);
             }
@@ -253,13 +291,11 @@ __device__ __forceinline__ void process_item(const ForceArgs &a, const LawConst 
             return;
         }
         if (!COUNT && LAW == LAW_LOGTAB) {
-            float dx[IPL], dy[IPL], dz[IPL], t[IPL];
+            float t[IPL];
             float4 c[IPL];
 #pragma unroll
-            for (int k = 0; k < IPL; ++k) {
-                dx[k] = pj.x - xi[k]; dy[k] = pj.y - yi[k]; dz[k] = pj.z - zi[k];
+            for (int k = 0; k < IPL; ++k)
                 c[k] = logtab_segment(soft_s<SOFT, !FOLD64>(dx[k], dy[k], dz[k], lc), lds, seg_mask, tab_shift, t[k]);
-            }
 #pragma unroll
             for (int k = 0; k < IPL; ++k) {
                 const float f = logtab_eval(c[k], t[k]);
@@ -271,12 +307,10 @@ __device__ __forceinline__ void process_item(const ForceArgs &a, const LawConst 
         }
 #pragma unroll
         for (int k = 0; k < IPL; ++k) {
-            if (COUNT) {
-                const float dx = pj.x - xi[k], dy = pj.y - yi[k], dz = pj.z - zi[k];
-                n_in += valid[k] && soft_s<SOFT, !FOLD64>(dx, dy, dz, lc) < 1.0f;
-            } else {
-                pair_poly<SOFT, !FOLD64>(xi[k], yi[k], zi[k], pj, lc, fx[k], fy[k], fz[k]);
-            }
+            if (COUNT)
+                n_in += valid[k] && soft_s<SOFT, !FOLD64>(dx[k], dy[k], dz[k], lc) < 1.0f;
+            else
+                pair_poly<SOFT, !FOLD64>(dx[k], dy[k], dz[k], lc, fx[k], fy[k], fz[k]);
         }
     };
     // FP32 partial sums fold into FP64 once at least 64 steps have gone into
@@ -292,16 +326,37 @@ __device__ __forceinline__ void process_item(const ForceArgs &a, const LawConst 
         }
         unfolded = 0;
     };
-    // one batch of n (a multiple of G) tile entries, the pointer stepping by
-    // G.  One loop serves every phase count: specialised copies per count
-    // measured slower, the kernel outgrowing the instruction cache.
+    // One batch of n (a multiple of G) tile entries; phase p takes entries
+    // p, p + G, p + 2G, ...  When L is a multiple of 4 every quad lies in one
+    // phase: per block of four steps each lane reads one entry, lane c of a
+    // quad entry (ub + c) G + phase, and step ub + u takes lane u's by a quad
+    // broadcast (one tile read per four steps, the pair arithmetic
+    // unchanged).  Other shapes read the tile every step, the pointer
+    // stepping by G.  One loop per form serves every phase count: specialised
+    // copies per count measured slower, the kernel outgrowing the
+    // instruction cache.
+    const float x0 = xi[0], y0 = yi[0], z0 = zi[0];
     auto run_batch = [&](int n) {
         const int ch = n / G;
-        const float4 *tp = tile + rphase;
+        if (quads) {
+            // n is a multiple of 4 G (batch sizes and the null padding of the
+            // last batch): ch is a multiple of 4 and every entry read, u G +
+            // phase <= n - 1 < 64, lies in the batch
+            for (int ub = 0; ub < ch; ub += 4) {
+                const float4 mine = tile[(ub + (lane & 3)) * G + rphase];
+                quad_step<0>(mine, x0, y0, z0, pairs_of);
+                quad_step<1>(mine, x0, y0, z0, pairs_of);
+                quad_step<2>(mine, x0, y0, z0, pairs_of);
+                quad_step<3>(mine, x0, y0, z0, pairs_of);
+            }
+        } else {
+            const float4 *tp = tile + rphase;
 #pragma unroll 4
-        for (int u = 0; u < ch; ++u) {
-            pairs_of(*tp);
-            tp += G;
+            for (int u = 0; u < ch; ++u) {
+                const float4 pj = *tp;
+                pairs_of(pj.x - x0, pj.y - y0, pj.z - z0);
+                tp += G;
+            }
         }
         unfolded += ch;
         if (FOLD64 && unfolded >= kWave)
@@ -416,7 +471,8 @@ __device__ __forceinline__ void process_item(const ForceArgs &a, const LawConst 
         wave_lds_sync();                               // directory reused by the next batch
     }
     if (cnt > 0) {
-        const int n = (cnt + G - 1) / G * G;
+        // cnt < nb, and nb is a multiple of bq: n <= nb <= 64
+        const int n = (cnt + bq - 1) / bq * bq;
         if (lane < n - cnt)
             tile[cnt + lane] = null_particle();
         wave_lds_sync();

@@ -110,7 +110,10 @@ def test_graph_replay_matches_eager(gpu_ctx, law):
     for _ in range(3):
         gpu_ctx.run_graph(pp.EXEC_QUEUE, nsteps=3, nreplay=2)
         rep = gpu_ctx.acc()
-        np.testing.assert_allclose(rep, eager, rtol=1e-12, atol=0)
+        # items of one target add into its FP64 accumulator by atomics in
+        # completion order, so a sum can differ in its last bits; an element
+        # that cancels to far below the largest keeps that absolute error
+        np.testing.assert_allclose(rep, eager, rtol=1e-12, atol=1e-13 * np.abs(eager).max())
         gpu_ctx.zero_acc()
         gpu_ctx.compute(pp.EXEC_QUEUE)
         gpu_ctx.sync()
@@ -177,7 +180,9 @@ def test_streamed_lists_accumulate(gpu_ctx):
     assert pyoracle.rel_err(a, ref)[1] <= P99_TOL
     ctx.compute()
     a2 = ctx.acc()
-    np.testing.assert_allclose(a2, 2 * a, rtol=1e-12, atol=1e-300)
+    # FP64 atomics add the items in completion order (last-bit differences,
+    # absolute on elements that cancel)
+    np.testing.assert_allclose(a2, 2 * a, rtol=1e-12, atol=1e-13 * np.abs(a).max())
 
 
 def test_autotune_keeps_results_and_picks_a_measured_config(gpu_ctx):
