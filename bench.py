@@ -40,7 +40,7 @@ METRIC = "G pair-interactions/s + % FP32 vector peak at p99 force err <=1e-5, 1 
 # HBM bytes per force launch (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, factors from a
 # 1 GiB copy kernel profiled in the same runs); see the named profile summary
 MEASURED_TRAFFIC = {
-    "clustered256": {"bytes": 6.557e10, "profile": "profiles/p2p_force_clustered256_traffic.md"},
+    "clustered256": {"bytes": 6.551e10, "profile": "profiles/p2p_force_clustered256_traffic.md"},
 }
 
 
@@ -377,6 +377,9 @@ def main():
             law = cfg.get("law", law)
             tuned = {"source": "photons_p2p/tuned.json", "signature": sig, "best": cfg}
             log(f"[bench] tuned.json {sig}: {cfg}")
+    if tuned is not None:
+        prep_ms = ctx.prepare()                    # the items of the tuned split
+        st = ctx.list_stats()
     n_list = st["npairs"]
     n_in = ctx.count_inside()
     log(f"[bench] prepared in {prep_ms:.1f} ms: rows={st['nrows']} items={st['nitems']} "

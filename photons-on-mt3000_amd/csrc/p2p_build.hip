@@ -254,16 +254,29 @@ __device__ __forceinline__ void row_i(const RowView &v, int tgt, int &ibeg, int 
     }
 }
 
-// pairs of every row (sum over its tasks of n_i * n_s)
-__global__ void k_row_cost(RowView v, int nrows, unsigned long long *__restrict__ cost)
+// pairs of every row (sum over its tasks of n_i * n_s), and the rows' n_s
+// summed into *staged (one atomic per wave; blockDim.x is a multiple of 64)
+__global__ void k_row_cost(RowView v, int nrows, unsigned long long *__restrict__ cost,
+                           unsigned long long *__restrict__ staged)
 {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= nrows)
-        return;
-    int ibeg, ni;
-    row_i(v, v.row_tgt[r], ibeg, ni);
-    const unsigned ns = v.nsx[v.row_off[r + 1]] - v.nsx[v.row_off[r]];
-    cost[r] = (unsigned long long)ns * (unsigned long long)ni;
+    unsigned long long ns = 0;
+    if (r < nrows) {
+        int ibeg, ni;
+        row_i(v, v.row_tgt[r], ibeg, ni);
+        ns = v.nsx[v.row_off[r + 1]] - v.nsx[v.row_off[r]];
+        cost[r] = ns * (unsigned long long)ni;
+    }
+    for (int o = 32; o > 0; o >>= 1)
+        ns += __shfl_xor(ns, o, kWave);
+    if ((threadIdx.x & (kWave - 1)) == 0 && ns > 0)
+        atomicAdd(staged, ns);
+}
+
+__global__ void k_zero_u64(unsigned long long *p)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0)
+        p[0] = 0ull;
 }
 
 // Shape of an item with c i-particles: L lanes per phase, G = 64 / L phases
@@ -306,13 +319,6 @@ __device__ __forceinline__ void chunk_of(int ni, int k, int q, int &beg, int &si
     beg = q * base + min(q, rem);
     size = base + (q < rem ? 1 : 0);
 }
-
-// A staged source particle costs a wave about as much as this many pair
-// evaluations (its frame shift, cull and compaction, the window walk and the
-// dependent loads of a source block).  Measured on the 1 M uneven leaf-pair
-// list: 8 -> 0.66 ms, 16 -> 0.62, 24 -> 0.614, 32 -> 0.62, 64 -> 0.63; the
-// clustered lists move by less than 0.5%.
-constexpr long long kStagePairs = 24;
 
 // Source ranges of a row: greedy cut when the running work of the largest
 // i-chunk, (n_i + kStagePairs) per source particle, would exceed `split`;
@@ -536,7 +542,8 @@ hipError_t rows_cost(const RowArgs &ra, unsigned long long *cost, unsigned long 
         return rocprim::reduce(nullptr, *temp_bytes, cost, total, 0ull, (size_t)ra.nrows,
                                rocprim::plus<unsigned long long>(), st);
     RowView v{ra.row_tgt, ra.row_off, ra.src, ra.rng, ra.nsx, ra.act_cnt, ra.act_off, ra.stage_ops, ra.ipl_max};
-    hipLaunchKernelGGL(k_row_cost, dim3(grid_of(ra.nrows)), dim3(256), 0, st, v, ra.nrows, cost);
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, total + 1);
+    hipLaunchKernelGGL(k_row_cost, dim3(grid_of(ra.nrows)), dim3(256), 0, st, v, ra.nrows, cost, total + 1);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess)
         return e;

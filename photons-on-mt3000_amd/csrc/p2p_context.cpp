@@ -1475,7 +1475,7 @@ static int items_build(pp2p_ctx *c, Space &sp, ListView &L, std::vector<int> *h_
     unsigned long long *row_cost = nullptr, *total = nullptr;
     int *item_count = nullptr, *item_off = nullptr;
     int st;
-    if ((st = sp.get(c->row_cost, nr, &row_cost)) || (st = sp.get(c->total, 1, &total)) ||
+    if ((st = sp.get(c->row_cost, nr, &row_cost)) || (st = sp.get(c->total, 2, &total)) ||
         (st = sp.get(c->item_count, (size_t)nr + 1, &item_count)) ||
         (st = sp.get(c->item_off, (size_t)nr + 1, &item_off)))
         return st;
@@ -1499,18 +1499,20 @@ static int items_build(pp2p_ctx *c, Space &sp, ListView &L, std::vector<int> *h_
     tb0 = tb1 = tb2 = tlen;
     HIPCHK(src_prefix(L.src, c->rng.p, L.ntask, L.nsx, tmp, &tb0, c->stream));
     HIPCHK(rows_cost(ra, row_cost, total, tmp, &tb1, c->stream));
-    unsigned long long pairs = 0;
-    HIPCHK(hipMemcpyAsync(&pairs, total, sizeof(pairs), hipMemcpyDeviceToHost, c->stream));
+    unsigned long long tot[2] = {0, 0};              // pairs, staged source particles
+    HIPCHK(hipMemcpyAsync(tot, total, sizeof(tot), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    L.pairs = pairs;
+    L.pairs = tot[0];
 
     long long split;
     if (c->tuning.split_pairs > 0) {
         split = c->tuning.split_pairs;
     } else {
-        // about 8 items per persistent wave, within [2^14, 2^22] pairs
+        // about 8 items per persistent wave of the modelled work (pairs and
+        // staging, as the cut counts it), within [2^14, 2^22]
         const long long waves = (long long)queue_waves(c);
-        split = (long long)(pairs / (unsigned long long)std::max(1ll, 8 * waves));
+        const unsigned long long work = tot[0] + (unsigned long long)kStagePairs * tot[1];
+        split = (long long)(work / (unsigned long long)std::max(1ll, 8 * waves));
         split = std::max(split, 1ll << 14);
         split = std::min(split, 1ll << 22);
     }
