@@ -2,7 +2,7 @@
 //
 // Force kernel: persistent waves pop work items (i-particles of one target
 // leaf against a range of its CSR source leaves) from a cost-sorted queue.
-// i-particles live in registers, up to four per lane; the source particles of
+// i-particles live in registers, up to three per lane; the source particles of
 // the item run through a wave-private LDS tile in windows of 64 that span
 // consecutive source leaves, each particle moved from its leaf's frame into
 // the target leaf's frame with an FP64 offset (minimum image when the box is
@@ -13,8 +13,9 @@
 // by DPP quad broadcasts folded into the first subtraction (the tile reads
 // were 40% of a step's LDS bytes on the linear law at three i-particles per
 // lane).  Pair arithmetic is FP32:
-// the LOGTAB law needs 13 FP32 and 3 integer VALU operations per pair and no
-// transcendental; FP32 partial sums fold into FP64 once per batch.
+// the LOGTAB law needs 13 FP32 and 3 integer VALU operations per pair, the
+// LINEAR law 10 and 3, neither a transcendental; FP32 partial sums fold into
+// FP64 after at least 64 steps.
 //
 // Reference semantics (what is computed, not how): P2P_core / P2P_core_1 /
 // P2P_kernel in src/gpu_kernel.impl.h:50-157, :260-362, :364-437.
