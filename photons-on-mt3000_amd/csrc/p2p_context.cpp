@@ -1617,6 +1617,13 @@ static ForceArgs force_args(pp2p_ctx *c, bool queue)
 }
 
 // enqueue the launches of one evaluation (no synchronisation; graph-safe)
+// QUEUE launches of fewer than 16 items per persistent wave deal the grid's
+// first items out instead of starting every wave on the queue counter
+static void set_deal(ForceArgs &a, int groups, int wpg)
+{
+    a.deal = (long long)a.nitems < 16ll * groups * wpg ? groups * wpg : 0;
+}
+
 static int enqueue_compute(pp2p_ctx *c, int exec)
 {
     const bool queue = exec == PP2P_EXEC_QUEUE;
@@ -1628,6 +1635,7 @@ static int enqueue_compute(pp2p_ctx *c, int exec)
     if (queue) {
         HIPCHK(launch_reset_counter(c->counter.p, c->stream));
         const int groups = std::min(queue_groups(c, v), (a.nitems + wpg - 1) / wpg);
+        set_deal(a, groups, wpg);
         HIPCHK(launch_force(a, c->law.lc, v, groups, c->stream));
     } else if (exec == PP2P_EXEC_GRID) {
         HIPCHK(launch_force(a, c->law.lc, v, (a.nitems + wpg - 1) / wpg, c->stream));
@@ -1702,7 +1710,9 @@ int pp2p_compute_noalloc(pp2p_ctx *c, long long ntask, const int *task_t, const 
     HIPCHK(hipEventRecord(c->ev_k0, c->stream));
     if (a.nitems > 0) {
         HIPCHK(launch_reset_counter(counter, c->stream));
-        HIPCHK(launch_force(a, c->law.lc, v, std::min(queue_groups(c, v), (a.nitems + wpg - 1) / wpg), c->stream));
+        const int groups = std::min(queue_groups(c, v), (a.nitems + wpg - 1) / wpg);
+        set_deal(a, groups, wpg);
+        HIPCHK(launch_force(a, c->law.lc, v, groups, c->stream));
     }
     HIPCHK(hipEventRecord(c->ev_k1, c->stream));
     c->kernel_timed = true;
@@ -1880,6 +1890,7 @@ int pp2p_count_inside(pp2p_ctx *c, long long *n_inside)
         ForceVariant v = variant(c, true);
         v.law = LAW_LOGTAB;                // the counting kernels have the table law's workgroup shape
         const int groups = std::min(queue_groups(c, v), (a.nitems + kWavesPerGroup - 1) / kWavesPerGroup);
+        set_deal(a, groups, kWavesPerGroup);
         HIPCHK(launch_count(a, c->law.lc, v, groups, c->stream));
     }
     HIPCHK(hipMemcpyAsync(&total, c->pair_count.p, sizeof(total), hipMemcpyDeviceToHost, c->stream));

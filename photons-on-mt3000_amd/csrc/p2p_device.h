@@ -97,6 +97,7 @@ struct ForceArgs {
     int nitems;
     int item_base;            // first item of this launch (GRID / PER_TARGET forms)
     unsigned *counter;        // queue head (QUEUE form)
+    int deal;                 // QUEUE: > 0: the grid's wave count, its first items dealt out (not from the counter)
     const float4 *pos;        // particle positions (scaled by 1/rc) in their leaf frame
     const double4 *ref;       // leaf frame origins (FP64, scaled by 1/rc)
     const int2 *rng;          // leaf (ip, np)
