@@ -26,7 +26,7 @@ TUNED = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned.json")
 
 GRID = {
     "ipl_max": [1, 2, 3],
-    "split_pairs": [0, 1 << 16, 1 << 18, 1 << 20, 1 << 22],
+    "split_pairs": [0, 1 << 15, 1 << 16, 1 << 17, 1 << 18, 1 << 19, 1 << 20, 1 << 22],
     "waves_per_cu": [0, 12, 8],
     "law": [LAW_TABLE, LAW_LINEAR, LAW_POLY],
 }
