@@ -181,7 +181,9 @@ def test_against_the_reference_kernel_restatement_exact_inputs(gpu_ctx):
     positions rounded to multiples of 2^-14 in a box of 100 (21 significant
     bits, so the reference's float positions and their differences are
     exact).  What remains of the reference's error is its table law and FP32
-    sums; the device must then agree with it to p99 <= 1e-5."""
+    sums, p99 about 1e-5 against the FP64 oracle on this list; the device
+    must agree with it to twice that (p99 <= 2e-5, the two errors added) and
+    be closer to the FP64 oracle than the restatement is."""
     w = W.config_plain(24)
     pos = np.round(w.pos * 2.0 ** 14) / 2.0 ** 14
     a = _ctx_run(gpu_ctx, w, pos=pos)
@@ -190,11 +192,14 @@ def test_against_the_reference_kernel_restatement_exact_inputs(gpu_ctx):
     p32 = np.ascontiguousarray(pos.T.astype(np.float32))
     assert np.array_equal(p32.T.astype(np.float64), pos)            # exact in FP32
     r32 = pyoracle.p2p_ref32(prm, p32, w.leaf_ip, w.leaf_np, csr).T.astype(np.float64)
+    f64 = pyoracle.p2p_f64(prm, pos, w.leaf_ip, w.leaf_np, csr)
     med, p99, mx = pyoracle.rel_err(a, r32)
-    m2, p2, x2 = pyoracle.rel_err(r32, pyoracle.p2p_f64(prm, pos, w.leaf_ip, w.leaf_np, csr))
+    m2, p2, x2 = pyoracle.rel_err(r32, f64)
+    m3, p3, x3 = pyoracle.rel_err(a, f64)
     print(f"exact inputs: device vs ref32 median {med:.2e} p99 {p99:.2e} max {mx:.2e}; "
-          f"ref32 vs f64 p99 {p2:.2e} max {x2:.2e}")
-    assert p99 <= 1e-5 and mx <= 1e-4
+          f"ref32 vs f64 p99 {p2:.2e} max {x2:.2e}; device vs f64 p99 {p3:.2e} max {x3:.2e}")
+    assert p99 <= 2e-5 and mx <= 2e-4
+    assert p3 < p2 and p3 <= 1e-5
 
 
 @pytest.mark.parametrize("fraction", [0.01, 0.1, 1.0])
