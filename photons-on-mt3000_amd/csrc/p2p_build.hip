@@ -338,11 +338,23 @@ __device__ int row_ranges(const RowView &v, int r, long long split, int tgt, int
     const int k0 = v.row_off[r], k1 = v.row_off[r + 1];
     int nr = 0, out = out0;
     int beg = k0;
+    // the row's chunks have two sizes at most (ni / k and one more): their
+    // shapes once per row, not once per item (the shape search divides in a
+    // loop of up to 63 steps)
+    int lanes_lo = 0, ipl_lo = 0, lanes_hi = 0, ipl_hi = 0;
+    if (EMIT) {
+        const int base = ni / k;
+        if (base > 0)
+            item_shape(base, v.ipl_max, lanes_lo, ipl_lo);
+        if (ni % k)
+            item_shape(base + 1, v.ipl_max, lanes_hi, ipl_hi);
+    }
     auto emit = [&](int s_beg, int s_end, long long ns) {
         for (int q = 0; q < k; ++q) {
-            int cb, cs, lanes, ipl;
+            int cb, cs;
             chunk_of(ni, k, q, cb, cs);
-            item_shape(cs, v.ipl_max, lanes, ipl);
+            const bool hi = cs > ni / k;
+            const int lanes = hi ? lanes_hi : lanes_lo, ipl = hi ? ipl_hi : ipl_lo;
             items[out] = Item{tgt, ibeg + cb, cs, r, s_beg, s_end, lanes, ipl};
             if (cost)
                 cost[out] = (unsigned)min(ns * ((long long)(kWave * ipl / (kWave / lanes)) + kStagePairs),
