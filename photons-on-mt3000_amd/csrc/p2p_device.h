@@ -77,7 +77,7 @@ struct alignas(16) Item {
     int s_beg;    // source range [s_beg, s_end) into src[]
     int s_end;
     int pad0;     // lanes per phase L (1..64); G = 64 / L phases
-    int pad1;     // i-particles per lane (IPL), 1..4
+    int pad1;     // i-particles per lane (IPL), 1..3
 };
 
 // Force-law constants, passed by value.
