@@ -113,6 +113,10 @@ void parallel_ranges(long long n, Fn fn)
 
 }  // namespace
 
+struct pp2p_ctx;
+template <typename Fill>
+static hipError_t staged_h2d(pp2p_ctx *c, void *dst, size_t units, size_t unit, size_t at, Fill fill);
+
 struct pp2p_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -506,6 +510,35 @@ int pp2p_set_particles_f64(pp2p_ctx *c, long long npart, const double *pos, int 
     return finish_particles(c, npart, nleaf);
 }
 
+// Host-to-device copy of `units` records of `unit` bytes through the
+// context's pinned buffer (from byte `at`, reserved by the caller) in four
+// pieces: fill(stage, u0, u1) writes records [u0, u1) to stage + u0 * unit
+// (called on up to 16 threads, each with its own sub-range), and the DMA of a
+// piece overlaps the filling of the next.  Copies straight from pageable
+// memory ran at about half the rate.  The buffer stays in use until the
+// stream has passed the copies.
+extern "C++" {
+template <typename Fill>
+static hipError_t staged_h2d(pp2p_ctx *c, void *dst, size_t units, size_t unit, size_t at, Fill fill)
+{
+    char *stage = static_cast<char *>(c->h_stage.p) + at;
+    constexpr int kPieces = 4;
+    for (int k = 0; k < kPieces; ++k) {
+        const size_t u0 = units * k / kPieces, u1 = units * (k + 1) / kPieces;
+        if (u1 <= u0)
+            continue;
+        parallel_ranges((long long)(u1 - u0), [&](long long i0, long long i1) {
+            fill(stage, u0 + (size_t)i0, u0 + (size_t)i1);
+        });
+        hipError_t e = hipMemcpyAsync(static_cast<char *>(dst) + u0 * unit, stage + u0 * unit, (u1 - u0) * unit,
+                                      hipMemcpyHostToDevice, c->stream);
+        if (e != hipSuccess)
+            return e;
+    }
+    return hipSuccess;
+}
+}  // extern "C++"
+
 int pp2p_set_particles_i32(pp2p_ctx *c, long long npart, const void *posi, size_t stride, double int2pos, int nleaf,
                            const int *leaf_ip, const int *leaf_np)
 {
@@ -521,23 +554,21 @@ int pp2p_set_particles_i32(pp2p_ctx *c, long long npart, const void *posi, size_
     HIPCHK(c->posi.reserve(std::max<long long>(3 * npart, 1)));
     HIPCHK(c->x3.reserve(std::max<long long>(3 * npart, 1)));
     if (npart) {
-        // one contiguous copy: records wider than the three coordinates are
-        // packed on the host first (as the reference repacks part[i].posi,
-        // src/gpu_entry.cu:85-105), in parallel, into a pinned buffer kept by
-        // the context; a pitched copy of npart 12-byte rows from pageable
-        // memory is orders of magnitude slower
-        const void *src = posi;
-        if (stride != 12) {
-            HIPCHK(c->h_stage.reserve(12 * (size_t)npart));
-            int *packed = static_cast<int *>(c->h_stage.p);
-            const unsigned char *b = static_cast<const unsigned char *>(posi);
-            parallel_ranges(npart, [&](long long i0, long long i1) {
-                for (long long i = i0; i < i1; ++i)
-                    std::memcpy(&packed[3 * (size_t)i], b + (size_t)i * stride, 12);
-            });
-            src = packed;
-        }
-        HIPCHK(hipMemcpyAsync(c->posi.p, src, 12 * (size_t)npart, hipMemcpyHostToDevice, c->stream));
+        // contiguous 12-byte rows through a pinned buffer kept by the context:
+        // records wider than the three coordinates are packed on the way (as
+        // the reference repacks part[i].posi, src/gpu_entry.cu:85-105); a
+        // pitched copy of npart 12-byte rows from pageable memory is orders of
+        // magnitude slower
+        HIPCHK(c->h_stage.reserve(12 * (size_t)npart));
+        const unsigned char *b = static_cast<const unsigned char *>(posi);
+        HIPCHK(staged_h2d(c, c->posi.p, (size_t)npart, 12, 0, [&](char *stage, size_t u0, size_t u1) {
+            if (stride == 12) {
+                std::memcpy(stage + 12 * u0, b + 12 * u0, 12 * (u1 - u0));
+                return;
+            }
+            for (size_t i = u0; i < u1; ++i)
+                std::memcpy(stage + 12 * i, b + i * stride, 12);
+        }));
         HIPCHK(launch_i32_to_f64(c->posi.p, 3 * npart, int2pos, false, c->x3.p, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));      // before the staging buffer is reused
     }
@@ -868,8 +899,16 @@ int pp2p_build_tree(pp2p_ctx *c, long long n, const int *posi, const int *tag, d
         HIPCHK(b->reserve(n));
     HIPCHK(c->tsmall.reserve(n + 1));
     HIPCHK(c->tsx.reserve(n + 1));
-    HIPCHK(hipMemcpyAsync(c->tposi.p, posi, sizeof(int) * 3 * n, hipMemcpyHostToDevice, sm));
-    HIPCHK(hipMemcpyAsync(c->ttag.p, tag, sizeof(int) * n, hipMemcpyHostToDevice, sm));
+    {
+        const size_t np3 = 3 * (size_t)n;
+        HIPCHK(c->h_stage.reserve(sizeof(int) * (np3 + (size_t)n)));
+        HIPCHK(staged_h2d(c, c->tposi.p, np3, sizeof(int), 0, [&](char *stage, size_t u0, size_t u1) {
+            std::memcpy(stage + sizeof(int) * u0, posi + u0, sizeof(int) * (u1 - u0));
+        }));
+        HIPCHK(staged_h2d(c, c->ttag.p, (size_t)n, sizeof(int), sizeof(int) * np3, [&](char *stage, size_t u0, size_t u1) {
+            std::memcpy(stage + sizeof(int) * u0, tag + u0, sizeof(int) * (u1 - u0));
+        }));
+    }
     int *const xs[3] = {c->tx0.p, c->tx1.p, c->tx2.p};
     const TreeScratch w{{c->tx0.p, c->tx1.p, c->tx2.p}, c->tidx.p, c->tsegof.p, c->tsmall.p, c->tsx.p, c->ttbl.p};
     HIPCHK(tree_init(c->tposi.p, n, xs, c->tidx.p, c->tsegof.p, sm));
