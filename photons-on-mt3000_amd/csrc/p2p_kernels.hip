@@ -6,7 +6,8 @@
 // the item run through a wave-private LDS tile in windows of 64 that span
 // consecutive source leaves, each particle moved from its leaf's frame into
 // the target leaf's frame with an FP64 offset (minimum image when the box is
-// periodic), culled against the item's bounding box and compacted.  Targets
+// periodic), culled against the item's bounding box and compacted, with the
+// loads of the next two windows in flight.  Targets
 // smaller than 64 lane slots split every batch of the tile over G lane groups
 // ("phases") that meet by a butterfly at the end.  When the phases are whole
 // quads a lane reads one tile entry per four steps and the steps share them
@@ -14,12 +15,15 @@
 // were 40% of a step's LDS bytes on the linear law at three i-particles per
 // lane).  Pair arithmetic is FP32:
 // the LOGTAB law needs 13 FP32 and 3 integer VALU operations per pair, the
-// LINEAR law 10 and 3, neither a transcendental; FP32 partial sums fold into
-// FP64 after at least 64 steps.
+// LINEAR law 10 and 3, neither a transcendental; a step's coefficient
+// gathers issue before the previous step's polynomials.  FP32 partial sums
+// fold into FP64 after at least 64 steps.
 //
 // Reference semantics (what is computed, not how): P2P_core / P2P_core_1 /
 // P2P_kernel in src/gpu_kernel.impl.h:50-157, :260-362, :364-437.
 #include "p2p_device.h"
+
+#include <type_traits>
 
 namespace pp2p {
 
@@ -263,6 +267,48 @@ __device__ __forceinline__ void process_item(const ForceArgs &a, const LawConst 
     for (int k = 0; k < IPL; ++k) {
         oxi[k] = xi[0] - xi[k]; oyi[k] = yi[0] - yi[k]; ozi[k] = zi[0] - zi[k];
     }
+    // A table-law step in two halves: `issue` forms the IPL pairs'
+    // differences and s and starts their coefficient gathers; `finish`
+    // evaluates the polynomials and accumulates.  The quad loop below runs
+    // the next step's issue before this step's finish, so a wave has a
+    // step's arithmetic to do while its gathers are in flight.
+    using Coef = typename std::conditional<LAW == LAW_LINEAR, float2, float4>::type;
+    struct Pend {
+        float dx[IPL], dy[IPL], dz[IPL], t[IPL];    // t: s (LINEAR) or s - s0 (LOGTAB)
+        Coef c[IPL];
+    };
+    auto issue = [&](float dx0, float dy0, float dz0, Pend &p) {
+#pragma unroll
+        for (int k = 0; k < IPL; ++k) {
+            p.dx[k] = k == 0 ? dx0 : dx0 + oxi[k];
+            p.dy[k] = k == 0 ? dy0 : dy0 + oyi[k];
+            p.dz[k] = k == 0 ? dz0 : dz0 + ozi[k];
+        }
+#pragma unroll
+        for (int k = 0; k < IPL; ++k) {
+            if constexpr (LAW == LAW_LINEAR) {
+                p.t[k] = soft_s<SOFT, !FOLD64>(p.dx[k], p.dy[k], p.dz[k], lc);
+                p.c[k] = linear_segment(p.t[k], lds);
+            } else {
+                p.c[k] = logtab_segment(soft_s<SOFT, !FOLD64>(p.dx[k], p.dy[k], p.dz[k], lc), lds, seg_mask,
+                                        tab_shift, p.t[k]);
+            }
+        }
+    };
+    auto finish = [&](const Pend &p) {
+#pragma unroll
+        for (int k = 0; k < IPL; ++k) {
+            float f;
+            if constexpr (LAW == LAW_LINEAR)
+                f = fmaf(p.c[k].y, p.t[k], p.c[k].x);
+            else
+                f = logtab_eval(p.c[k], p.t[k]);
+            fx[k] = fmaf(p.dx[k], f, fx[k]);
+            fy[k] = fmaf(p.dy[k], f, fy[k]);
+            fz[k] = fmaf(p.dz[k], f, fz[k]);
+        }
+    };
+    constexpr bool kPipe = !COUNT && LAW != LAW_POLY;
     // the IPL pairs of one j given its differences to slot 0: all table
     // addresses first, then the gathers, then the polynomials, so that the
     // gathers of a step are in flight together
@@ -339,7 +385,26 @@ __device__ __forceinline__ void process_item(const ForceArgs &a, const LawConst 
     const float x0 = xi[0], y0 = yi[0], z0 = zi[0];
     auto run_batch = [&](int n) {
         const int ch = n / G;
-        if (quads) {
+        if (kPipe && quads) {
+            // as below, software-pipelined: step u + 1's gathers issue before
+            // step u's polynomials, and each block reads the next block's
+            // entry first (on the last block an entry past the batch, at most
+            // n + 4 G - 1 <= 127 < kTileCap: the wave's own tile, unused)
+            float4 mine = tile[(lane & 3) * G + rphase];
+            for (int ub = 0; ub < ch; ub += 4) {
+                const float4 next = tile[(ub + 4 + (lane & 3)) * G + rphase];
+                Pend A, B;
+                issue(quad_bcast<0>(mine.x) - x0, quad_bcast<0>(mine.y) - y0, quad_bcast<0>(mine.z) - z0, A);
+                issue(quad_bcast<1>(mine.x) - x0, quad_bcast<1>(mine.y) - y0, quad_bcast<1>(mine.z) - z0, B);
+                finish(A);
+                issue(quad_bcast<2>(mine.x) - x0, quad_bcast<2>(mine.y) - y0, quad_bcast<2>(mine.z) - z0, A);
+                finish(B);
+                issue(quad_bcast<3>(mine.x) - x0, quad_bcast<3>(mine.y) - y0, quad_bcast<3>(mine.z) - z0, B);
+                finish(A);
+                finish(B);
+                mine = next;
+            }
+        } else if (quads) {
             // n is a multiple of 4 G (batch sizes and the null padding of the
             // last batch): ch is a multiple of 4 and every entry read, u G +
             // phase <= n - 1 < 64, lies in the batch
@@ -369,8 +434,9 @@ __device__ __forceinline__ void process_item(const ForceArgs &a, const LawConst 
     // sizes in registers, its FP64 frame offset in the wave's LDS directory.
     // Particles are staged in windows of 64 that run over the concatenated
     // sources.  The sources that meet a window are found by a wave-uniform
-    // walk over the prefix (lane reads, no memory round trip), and the next
-    // window is prefetched while the current one is staged and computed.
+    // walk over the prefix (lane reads, no memory round trip), and the loads
+    // of the next two windows run while the current one is staged and
+    // computed.
     for (int sb = it.s_beg; sb < it.s_end; sb += kWave) {
         const int nsrc = min(kWave, it.s_end - sb);
         int m_ip = 0, m_np = 0;
@@ -418,20 +484,34 @@ __device__ __forceinline__ void process_item(const ForceArgs &a, const LawConst 
             }
             cur = sc;
         };
-        int J = lane;
-        int ks, bs;
-        locate(0, ks, bs);
-        float4 qv = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (J < total)
-            qv = a.pos[bs + J];
-        for (int w0 = 0; w0 < total; w0 += kWave) {
-            // stage the current window: target frame, cull, compact
-            const bool has = J < total;
-            float4 w = qv;
+        // Two windows in flight: the even windows load into qA, the odd ones
+        // into qB, each refilled with the window two ahead once staged, so a
+        // load has two windows of batches to arrive in.  Every window issues
+        // its refill, also past the block's end, so that the loads in flight
+        // are the same on every path and a staging waits for its own load
+        // only.  Lanes past the end load a particle of the block they never
+        // use: index min(w + lane, total - 1) from the base of the source
+        // located there.  Such a lane may land on a trailing empty source,
+        // whose base makes the index its first particle minus one: -1 when
+        // that empty source starts at particle 0, hence the clamp at 0 (with
+        // total = 0 the index is max(first particle - 1, 0); the item's
+        // target has particles, so the array is not empty).
+        int ksA, ksB, bs;
+        auto window_load = [&](int w, int &ks) {
+            locate(w, ks, bs);
+            return a.pos[max(bs + min(w + lane, total - 1), 0)];
+        };
+        float4 qA = window_load(0, ksA);
+        float4 qB = window_load(kWave, ksB);
+        // stage window w0 from q (target frame, cull, compact), then start
+        // the load of window w0 + 128 into q
+        auto stage = [&](int w0, float4 &q, int &ks) {
+            const bool has = w0 + lane < total;
+            float4 w = q;
             if (FOLD64) {                              // every lane's ks names a source of the block
-                w.x = (float)((double)qv.x + dir.ox[ks]);
-                w.y = (float)((double)qv.y + dir.oy[ks]);
-                w.z = (float)((double)qv.z + dir.oz[ks]);
+                w.x = (float)((double)q.x + dir.ox[ks]);
+                w.y = (float)((double)q.y + dir.oy[ks]);
+                w.z = (float)((double)q.z + dir.oz[ks]);
             }
             const float ex = fmaxf(fabsf(w.x - cx) - hx, 0.f);
             const float ey = fmaxf(fabsf(w.y - cy) - hy, 0.f);
@@ -441,20 +521,11 @@ __device__ __forceinline__ void process_item(const ForceArgs &a, const LawConst 
             if (keep)
                 tile[cnt + __popcll(m & below)] = make_float4(w.x, w.y, w.z, 0.f);
             cnt += __popcll(m);
-            // prefetch the next window now, so that its load overlaps the batches
-            // below; lanes past the end load a particle of the block they never
-            // use.  Such a lane may land on a trailing empty source, whose base
-            // makes the index its first particle minus one: -1 when that empty
-            // source starts at particle 0, hence the clamp.
-            const int J2 = J + kWave;
-            if (w0 + kWave < total) {
-                locate(w0 + kWave, ks, bs);
-                qv = a.pos[max(bs + min(J2, total - 1), 0)];
-            }
-            J = J2;
-            wave_lds_sync();
-            // cnt < nb before the window was added, so cnt < nb + 64 <= kTileCap
-            // and every remainder below is < 64 (one lane per entry)
+            q = window_load(w0 + 2 * kWave, ks);
+        };
+        // cnt < nb before a window is added, so cnt < nb + 64 <= kTileCap and
+        // every remainder below is < 64 (one lane per entry)
+        auto drain = [&]() {
             while (cnt >= nb) {
                 run_batch(nb);
                 wave_lds_sync();
@@ -468,6 +539,19 @@ __device__ __forceinline__ void process_item(const ForceArgs &a, const LawConst 
                 cnt = rem;
                 wave_lds_sync();
             }
+        };
+        // Unrolled by two with one exit at the bottom, so that at the top of
+        // every pass qA's load is the older of the two in flight and the wait
+        // before its staging leaves qB's outstanding.  A pass whose second
+        // window starts at or past the end stages it with no lane in range:
+        // it keeps nothing.
+        for (int w0 = 0; w0 < total; w0 += 2 * kWave) {
+            stage(w0, qA, ksA);
+            wave_lds_sync();
+            drain();
+            stage(w0 + kWave, qB, ksB);
+            wave_lds_sync();
+            drain();
         }
         wave_lds_sync();                               // directory reused by the next batch
     }
