@@ -5,10 +5,11 @@
 // i-particles live in registers, up to three per lane; the source particles of
 // the item run through a wave-private LDS tile in windows of 64 that span
 // consecutive source leaves, each particle moved from its leaf's frame into
-// the target leaf's frame with an FP64 offset (minimum image when the box is
-// periodic), culled against the item's bounding box and compacted, with the
-// loads of the next two windows in flight.  Targets
-// smaller than 64 lane slots split every batch of the tile over G lane groups
+// the target leaf's frame by an offset formed in FP64 per leaf pair (minimum
+// image when the box is periodic) and added as an FP32 pair hi + lo, culled
+// against the item's bounding box and compacted, with the loads of the next
+// two windows in flight.  Targets smaller than 64 lane slots split every
+// batch of the tile over G lane groups
 // ("phases") that meet by a butterfly at the end.  When the phases are whole
 // quads a lane reads one tile entry per four steps and the steps share them
 // by DPP quad broadcasts folded into the first subtraction (the tile reads
@@ -37,9 +38,11 @@ __device__ __forceinline__ void wave_lds_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
-// Per-wave LDS directory of up to 64 source leaves (see process_item).
+// Per-wave LDS directory of up to 64 source leaves (see process_item): the
+// frame offset of each (target frame - source frame), formed in FP64 with the
+// minimum image and kept as an unevaluated FP32 sum hi + lo (x: hi, y: lo).
 struct SrcDir {
-    double ox[kWave], oy[kWave], oz[kWave];   // FP64 frame offsets (target frame - source frame)
+    float2 ox[kWave], oy[kWave], oz[kWave];
 };
 static_assert(sizeof(SrcDir) <= kLdsDirBytes, "source directory size");
 
@@ -459,9 +462,13 @@ __device__ __forceinline__ void process_item(const ForceArgs &a, const LawConst 
         const int pre = wave_incl_scan(m_np, lane);          // sources 0..lane
         const int total = __builtin_amdgcn_readlane(pre, kWave - 1);
         const int base = m_ip - (pre - m_np);               // pos index of concatenated position 0, source lane
-        dir.ox[lane] = m_ox;
-        dir.oy[lane] = m_oy;
-        dir.oz[lane] = m_oz;
+        auto hi_lo = [](double v) {
+            const float h = (float)v;
+            return make_float2(h, (float)(v - (double)h));
+        };
+        dir.ox[lane] = hi_lo(m_ox);
+        dir.oy[lane] = hi_lo(m_oy);
+        dir.oz[lane] = hi_lo(m_oz);
         wave_lds_sync();
 
         // source and pos-index base of position w0 + lane; `cur` (uniform) is
@@ -509,9 +516,16 @@ __device__ __forceinline__ void process_item(const ForceArgs &a, const LawConst 
             const bool has = w0 + lane < total;
             float4 w = q;
             if (FOLD64) {                              // every lane's ks names a source of the block
-                w.x = (float)((double)q.x + dir.ox[ks]);
-                w.y = (float)((double)q.y + dir.oy[ks]);
-                w.z = (float)((double)q.z + dir.oz[ks]);
+                // (x + lo) + hi: the rounding of x + lo is that of a coordinate
+                // inside the source leaf, the second that of the result, so
+                // the shift stays within about one FP32 rounding of the
+                // target-frame coordinate (an FP64 sum rounded once: half of it)
+                // at two FP32 additions per axis instead of two conversions and
+                // an FP64 addition
+                const float2 ox = dir.ox[ks], oy = dir.oy[ks], oz = dir.oz[ks];
+                w.x = (q.x + ox.y) + ox.x;
+                w.y = (q.y + oy.y) + oy.x;
+                w.z = (q.z + oz.y) + oz.x;
             }
             const float ex = fmaxf(fabsf(w.x - cx) - hx, 0.f);
             const float ey = fmaxf(fabsf(w.y - cy) - hy, 0.f);
