@@ -40,7 +40,7 @@ METRIC = "G pair-interactions/s + % FP32 vector peak at p99 force err <=1e-5, 1 
 # HBM bytes per force launch (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, factors from a
 # 1 GiB copy kernel profiled in the same runs); see the named profile summary
 MEASURED_TRAFFIC = {
-    "clustered256": {"bytes": 6.708e10, "profile": "profiles/p2p_force_clustered256_traffic.md"},
+    "clustered256": {"bytes": 6.706e10, "profile": "profiles/p2p_force_clustered256_traffic.md"},
 }
 
 
