@@ -218,3 +218,59 @@ def test_device_csr_equals_the_stable_sort(gpu_ctx, path):
     # and the same as numpy's stable argsort of the targets
     o = np.argsort(tt, kind="stable")
     np.testing.assert_array_equal(c["src"], ts[o])
+
+
+def _block_edge_case():
+    """A target whose CSR sources fall in 64-source blocks of 0, 64, 65, 128
+    and (partial block) 129 particles, with empty leaves at particle 0, in the
+    middle and at the end of the particle array; a second target whose
+    sources are empty leaves only (and itself)."""
+    g = np.random.default_rng(17)
+    rc = 2 * RS * 2.8125
+    sizes = [20, 5] + [1] * 64 + [1] * 63 + [2] + [64, 64] + [129]
+    pos = np.concatenate([(g.random(3) * 0.8 * rc) + (g.random((n, 3)) - 0.5) * 1.0 for n in sizes])
+    npl = np.asarray(sizes, np.int32)
+    ip = np.concatenate([[0], np.cumsum(npl)[:-1]]).astype(np.int32)
+    npart = int(npl.sum())
+    full = ip.size
+    # empty leaves: 64 + 62 + 10 of them, at particle 0, mid-array and the end
+    e_at = np.array([0, npart // 2, npart] * 46, np.int32)[:136]
+    ip = np.concatenate([ip, e_at])
+    npl = np.concatenate([npl, np.zeros(e_at.size, np.int32)])
+    E = list(range(full, full + e_at.size))
+    singles = list(range(2, 66))
+    b2 = list(range(66, 129)) + [129]                         # 63 singles + the 2-particle leaf
+    b3 = [130, 131] + E[64:126]                               # two 64-particle leaves + 62 empty
+    b4 = [132, E[126], E[0]]                                  # 129 particles, then two empty leaves
+    t1 = E[:64] + singles + b2 + b3 + b4 + [0]
+    t2 = E[126:136] + E[:64] + [1]
+    tt = np.array([0] * len(t1) + [1] * len(t2), np.int32)
+    ts = np.array(t1 + t2, np.int32)
+    # per-block particle totals of target 0's sources, as the kernel sees them
+    tot = [int(npl[ts[:len(t1)][k:k + 64]].sum()) for k in range(0, len(t1), 64)]
+    assert tot[:4] == [0, 64, 65, 128] and tot[4] == 129 + 20
+    # a window load past a block's end clamps its index to [0, npart - 1]:
+    # from an empty leaf at ip the index is max(ip - 1, 0)
+    assert all(0 <= max(int(x) - 1, 0) <= npart - 1 for x in e_at)
+    return pos, ip, npl, tt, ts
+
+
+@pytest.mark.parametrize("law", [pp.LAW_TABLE, pp.LAW_LINEAR], ids=["table", "linear"])
+def test_source_blocks_of_zero_and_window_multiples(gpu_ctx, law):
+    """Source blocks of 64 leaves holding no particle at all, exactly one or
+    two windows, one window and one particle, and an odd number of windows
+    with trailing empty leaves: the window loop stages two windows per pass
+    and refills past a block's end from clamped indices.  Forces equal the
+    oracle's; the target whose sources are all empty gets only its self
+    pairs."""
+    pos, ip, npl, tt, ts = _block_edge_case()
+    ref = _oracle(pos, ip, npl, tt, ts)
+    for exec_form in (pp.EXEC_QUEUE, pp.EXEC_GRID):
+        print(f"case: source blocks law={law} exec={exec_form}", flush=True)
+        gpu_ctx.set_tuning()
+        gpu_ctx.set_law(pp.ForceLaw(eps=EPS, rs=RS, softening=1, box=BOX, law=law))
+        gpu_ctx.set_particles(pos, ip, npl)
+        gpu_ctx.set_tasks(tt, ts)
+        gpu_ctx.compute(exec_form)
+        a = gpu_ctx.acc()
+        _check(a, ref)
